@@ -1,0 +1,1111 @@
+/*
+ * CPU oracle for the linear SWE RK4 step (Ch staggering, SBP 21 / 42).
+ * TEST INFRASTRUCTURE ONLY -- see swe_oracle.h for scope and citations.
+ *
+ * The restatement keeps the reference's structure on purpose: halo-padded
+ * per-panel arrays, an explicit halo fill before every operator that reads
+ * across a panel edge, then 1-D SBP sweeps along rows and columns.  It is
+ * deliberately simple and scalar; speed is not its job.
+ */
+#include "swe_oracle.h"
+
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define HALO 4 /* widest read across an edge: c2i SAT depth 3 */
+
+/* Planted mistakes for tests that prove the checks have power (0 = none):
+ * 1 flip the sign of rotated Ch-vector halo entries, 2 drop the metric term
+ * of the covariant edge sync, 3 read the depth-2 xy halo instead of the
+ * duplicate in the gradient SAT, 4 perturb one 4th-order c2i closure
+ * coefficient, 5 skip the div projection. */
+static int g_mutation = 0;
+void swo_set_mutation(int m) { g_mutation = m; }
+static int g_no_sat = 0; /* set only inside swo_line_op */
+
+/* ------------------------------------------------------------------ */
+/* cube topology: topology/cubed_sphere_topology_mod.f90 L29-55.       */
+/* Panel frames on the unit cube: r = corner of point (1,1), ex / ey = */
+/* directions of increasing i / j, n = inward normal.                   */
+/* ------------------------------------------------------------------ */
+static const int T_EX[6][3] = {{-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 0, 1}, {1, 0, 0}, {1, 0, 0}};
+static const int T_EY[6][3] = {{0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, -1}};
+static const int T_N[6][3] = {{0, 0, 1}, {-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
+static const int T_R[6][3] = {{1, 0, 0}, {1, 0, 1}, {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 1, 1}};
+
+static int idot(const int *a, const int *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* ------------------------------------------------------------------ */
+/* halo-padded field: index (p, i, j), i in [1-HALO, nx+HALO]           */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double *d;
+    int nx, ny, sx, sy;
+} fld;
+
+static void fld_init(fld *f, int nx, int ny)
+{
+    f->nx = nx;
+    f->ny = ny;
+    f->sx = nx + 2 * HALO;
+    f->sy = ny + 2 * HALO;
+    f->d = (double *)calloc((size_t)6 * f->sx * f->sy, sizeof(double));
+}
+static void fld_free(fld *f) { free(f->d); }
+static inline double *AT(const fld *f, int p, int i, int j)
+{
+    return f->d + (size_t)p * f->sx * f->sy + (size_t)(j - 1 + HALO) * f->sx + (i - 1 + HALO);
+}
+static void fld_load(fld *f, const double *src)
+{
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= f->ny; j++)
+            for (int i = 1; i <= f->nx; i++)
+                *AT(f, p, i, j) = src[((size_t)p * f->ny + (j - 1)) * f->nx + (i - 1)];
+}
+static void fld_store(const fld *f, double *dst)
+{
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= f->ny; j++)
+            for (int i = 1; i <= f->nx; i++)
+                dst[((size_t)p * f->ny + (j - 1)) * f->nx + (i - 1)] = *AT(f, p, i, j);
+}
+
+/* ------------------------------------------------------------------ */
+/* halo maps                                                           */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int tp, ti, tj, tc; /* target panel, index, component (0 u-type, 1 v-type) */
+    int sp, si, sj, sc; /* source */
+    int sign;
+} hentry;
+
+typedef struct {
+    hentry *e;
+    int n;
+} hmap;
+
+/* axis types: 0 = vertex (N+1 points), 1 = centre (N points) */
+typedef struct {
+    int xt, yt;
+} mtype;
+static const mtype MT_XY = {0, 0}, MT_U = {1, 0}, MT_V = {0, 1};
+
+static int npts_axis(int N, int t) { return t == 0 ? N + 1 : N; }
+
+/*
+ * Unfold a target halo point across one panel edge (the index rule of
+ * topology/cubed_sphere_topology_mod.f90 transform_index L80-148 and the
+ * component rule of parallel/buffer_mod.f90 pack_to_buf_vec L171-277):
+ * the neighbour face is folded flat onto the target's plane; halo depth d
+ * counts points of the target's axis type into the neighbour from the
+ * shared edge, so for vertex-type axes d = 1 is the neighbour's duplicate
+ * of the edge point.  Components follow the folding isometry:
+ * target_comp = M * source_comp with M_ab = (unfolded target axis a) .
+ * (source axis b).
+ * side: 0 left (i<1), 1 right (i>nx), 2 bottom (j<1), 3 top (j>ny).
+ * t = 1-based along-edge index in the target's indexing.
+ */
+static void unfold(int N, int p, int side, int d, int t, mtype mt, int *q_out,
+                   int *si, int *sj, int M[2][2], mtype *smt)
+{
+    const int L = 2 * N; /* cube edge in half-steps */
+    int X[3], ex_u[3], ey_u[3];
+    int tang_type = (side < 2) ? mt.yt : mt.xt;
+    int norm_type = (side < 2) ? mt.xt : mt.yt;
+    int s = tang_type == 0 ? 2 * (t - 1) : 2 * t - 1;    /* along-edge position */
+    int dist = norm_type == 0 ? 2 * (d - 1) : 2 * d - 1; /* depth into neighbour */
+    int nq[3];
+    for (int k = 0; k < 3; k++) {
+        int R = L * T_R[p][k];
+        switch (side) {
+        case 0:
+            X[k] = R + s * T_EY[p][k] + dist * T_N[p][k];
+            ex_u[k] = -T_N[p][k];
+            ey_u[k] = T_EY[p][k];
+            nq[k] = T_EX[p][k];
+            break;
+        case 1:
+            X[k] = R + L * T_EX[p][k] + s * T_EY[p][k] + dist * T_N[p][k];
+            ex_u[k] = T_N[p][k];
+            ey_u[k] = T_EY[p][k];
+            nq[k] = -T_EX[p][k];
+            break;
+        case 2:
+            X[k] = R + s * T_EX[p][k] + dist * T_N[p][k];
+            ex_u[k] = T_EX[p][k];
+            ey_u[k] = -T_N[p][k];
+            nq[k] = T_EY[p][k];
+            break;
+        default:
+            X[k] = R + L * T_EY[p][k] + s * T_EX[p][k] + dist * T_N[p][k];
+            ex_u[k] = T_EX[p][k];
+            ey_u[k] = T_N[p][k];
+            nq[k] = -T_EY[p][k];
+            break;
+        }
+    }
+    int q = -1;
+    for (int k = 0; k < 6; k++)
+        if (T_N[k][0] == nq[0] && T_N[k][1] == nq[1] && T_N[k][2] == nq[2]) q = k;
+    int rel[3];
+    for (int k = 0; k < 3; k++) rel[k] = X[k] - L * T_R[q][k];
+    int xq = idot(rel, T_EX[q]), yq = idot(rel, T_EY[q]);
+    M[0][0] = idot(ex_u, T_EX[q]);
+    M[0][1] = idot(ex_u, T_EY[q]);
+    M[1][0] = idot(ey_u, T_EX[q]);
+    M[1][1] = idot(ey_u, T_EY[q]);
+    if (M[0][0] != 0) {
+        *smt = mt;
+    } else {
+        smt->xt = mt.yt;
+        smt->yt = mt.xt;
+    }
+    *si = smt->xt == 0 ? xq / 2 + 1 : (xq + 1) / 2;
+    *sj = smt->yt == 0 ? yq / 2 + 1 : (yq + 1) / 2;
+    *q_out = q;
+}
+
+static void hmap_push(hmap *m, hentry e)
+{
+    m->e = (hentry *)realloc(m->e, sizeof(hentry) * (m->n + 1));
+    m->e[m->n++] = e;
+}
+
+/* xy points, sides in {0..3}, depth 1..depth, both components recorded */
+static void build_xy_map(hmap *m, int N, int depth, int vector)
+{
+    for (int p = 0; p < 6; p++)
+        for (int side = 0; side < 4; side++)
+            for (int d = 1; d <= depth; d++)
+                for (int t = 1; t <= N + 1; t++) {
+                    int q, si, sj, M[2][2];
+                    mtype smt;
+                    unfold(N, p, side, d, t, MT_XY, &q, &si, &sj, M, &smt);
+                    int ti, tj;
+                    if (side == 0) { ti = 1 - d; tj = t; }
+                    else if (side == 1) { ti = N + 1 + d; tj = t; }
+                    else if (side == 2) { ti = t; tj = 1 - d; }
+                    else { ti = t; tj = N + 1 + d; }
+                    if (!vector) {
+                        hentry e = {p, ti, tj, 0, q, si, sj, 0, 1};
+                        hmap_push(m, e);
+                    } else {
+                        for (int c = 0; c < 2; c++) {
+                            int sc = M[c][0] != 0 ? 0 : 1;
+                            hentry e = {p, ti, tj, c, q, si, sj, sc, M[c][sc]};
+                            hmap_push(m, e);
+                        }
+                    }
+                }
+}
+
+/* Ch vector: u (mesh y) across left/right, v (mesh x) across bottom/top;
+ * parallel/exchange_factory_mod.f90 L369-672 with halo width hw+1 along the
+ * vertex-type axis, restricted to the rows the div SAT reads. */
+static void build_ch_map(hmap *m, int N, int depth)
+{
+    for (int p = 0; p < 6; p++)
+        for (int side = 0; side < 4; side++) {
+            mtype mt = side < 2 ? MT_U : MT_V;
+            int c = side < 2 ? 0 : 1;
+            for (int d = 1; d <= depth; d++)
+                for (int t = 1; t <= N + 1; t++) {
+                    int q, si, sj, M[2][2];
+                    mtype smt;
+                    unfold(N, p, side, d, t, mt, &q, &si, &sj, M, &smt);
+                    int ti, tj;
+                    if (side == 0) { ti = 1 - d; tj = t; }
+                    else if (side == 1) { ti = N + d; tj = t; }
+                    else if (side == 2) { ti = t; tj = 1 - d; }
+                    else { ti = t; tj = N + d; }
+                    int sc = M[c][0] != 0 ? 0 : 1;
+                    int sg = M[c][sc];
+                    if (g_mutation == 1 && M[0][0] == 0) sg = -sg;
+                    hentry e = {p, ti, tj, c, q, si, sj, sc, sg};
+                    hmap_push(m, e);
+                }
+        }
+}
+
+static void apply_map(const hmap *m, fld *a, fld *b)
+{
+    fld *F[2] = {a, b};
+    /* copy sources first: targets are halo points, sources interior */
+    for (int k = 0; k < m->n; k++) {
+        const hentry *e = &m->e[k];
+        *AT(F[e->tc], e->tp, e->ti, e->tj) = e->sign * *AT(F[e->sc], e->sp, e->si, e->sj);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* 1-D SBP operators on strided lines.  f points at index 0.           */
+/* ------------------------------------------------------------------ */
+#define L1(f, st, k) ((f)[(ptrdiff_t)(k) * (st)])
+
+/* free parameters of the 4th-order staggered interpolation,
+ * sbp_interp_i2c_42_mod.f90 L18-19 */
+static const double C13 = 0.1264926207034815, C14 = -0.029892648949074313;
+
+/* interfaces -> centres derivative, n centres out, reads f[0..n+2]
+ * (f[0], f[n+2] = neighbour's duplicates of the edge points)
+ * sbp_diff_i2c_21_mod.f90 L24-144, sbp_diff_i2c_42_mod.f90 L29-274 */
+static void d_i2c(int order, const double *f, ptrdiff_t fs, int n, double *o, ptrdiff_t os)
+{
+    if (order == 21) {
+        for (int k = 1; k <= n; k++) L1(o, os, k) = L1(f, fs, k + 1) - L1(f, fs, k);
+        if (g_no_sat) return;
+        double jl = L1(f, fs, 1) - L1(f, fs, 0);
+        double jr = L1(f, fs, n + 2) - L1(f, fs, n + 1);
+        /* 0.5*R/h with R = (3/2,-1/2), h = (1,1) */
+        L1(o, os, 1) += 0.75 * jl;
+        L1(o, os, 2) += -0.25 * jl;
+        L1(o, os, n) += 0.75 * jr;
+        L1(o, os, n - 1) += -0.25 * jr;
+        return;
+    }
+    static const double Q[3][5] = {{-79.0 / 78, 27.0 / 26, -1.0 / 26, 1.0 / 78, 0},
+                                   {2.0 / 21, -9.0 / 7, 9.0 / 7, -2.0 / 21, 0},
+                                   {1.0 / 75, 0, -27.0 / 25, 83.0 / 75, -1.0 / 25}};
+    for (int r = 0; r < 3; r++) {
+        double sl = 0, sr = 0;
+        for (int k = 0; k < 5; k++) {
+            sl += Q[r][k] * L1(f, fs, 1 + k);
+            sr += Q[r][k] * L1(f, fs, n + 1 - k);
+        }
+        L1(o, os, 1 + r) = sl;
+        L1(o, os, n - r) = -sr;
+    }
+    for (int k = 4; k <= n - 3; k++)
+        L1(o, os, k) = (27.0 * (L1(f, fs, k + 1) - L1(f, fs, k)) - (L1(f, fs, k + 2) - L1(f, fs, k - 1))) / 24.0;
+    if (g_no_sat) return;
+    static const double H[3] = {13.0 / 12, 7.0 / 8, 25.0 / 24};
+    static const double R[3] = {15.0 / 8, -5.0 / 4, 3.0 / 8};
+    double jl = L1(f, fs, 1) - L1(f, fs, 0);
+    double jr = L1(f, fs, n + 2) - L1(f, fs, n + 1);
+    for (int r = 0; r < 3; r++) {
+        L1(o, os, 1 + r) += 0.5 * R[r] / H[r] * jl;
+        L1(o, os, n - r) += 0.5 * R[r] / H[r] * jr;
+    }
+}
+
+/* centres -> interfaces derivative, n centres in, n+1 interfaces out,
+ * SAT reads f[1-depth..0] and f[n+1..n+depth] (neighbour's normal flux)
+ * sbp_diff_c2i_21_mod.f90 L24-146, sbp_diff_c2i_42_mod.f90 L32-251 */
+static void d_c2i(int order, const double *f, ptrdiff_t fs, int n, double *o, ptrdiff_t os)
+{
+    int nx = n + 1;
+    if (order == 21) {
+        L1(o, os, 1) = L1(f, fs, 2) - L1(f, fs, 1);
+        for (int k = 2; k <= nx - 1; k++) L1(o, os, k) = L1(f, fs, k) - L1(f, fs, k - 1);
+        L1(o, os, nx) = L1(f, fs, nx - 1) - L1(f, fs, nx - 2);
+        if (g_no_sat) return;
+        /* 0.5/h11 with h11 = 1/2; R = (3/2, -1/2) */
+        L1(o, os, 1) += 1.0 * (1.5 * (L1(f, fs, 1) - L1(f, fs, 0)) - 0.5 * (L1(f, fs, 2) - L1(f, fs, -1)));
+        L1(o, os, nx) -= 1.0 * (1.5 * (L1(f, fs, nx - 1) - L1(f, fs, nx)) -
+                                0.5 * (L1(f, fs, nx - 2) - L1(f, fs, nx + 1)));
+        return;
+    }
+    double Q[4][5] = {{-2.0, 3.0, -1.0, 0, 0},
+                      {-1.0, 1.0, 0, 0, 0},
+                      {1.0 / 24, -9.0 / 8, 9.0 / 8, -1.0 / 24, 0},
+                      {-1.0 / 71, 6.0 / 71, -83.0 / 71, 81.0 / 71, -3.0 / 71}};
+    if (g_mutation == 4) Q[3][2] = -82.0 / 71;
+    for (int r = 0; r < 4; r++) {
+        double sl = 0, sr = 0;
+        for (int k = 0; k < 5; k++) {
+            sl += Q[r][k] * L1(f, fs, 1 + k);
+            sr += Q[r][k] * L1(f, fs, nx - 1 - k);
+        }
+        L1(o, os, 1 + r) = sl;
+        L1(o, os, nx - r) = -sr;
+    }
+    for (int k = 5; k <= nx - 4; k++)
+        L1(o, os, k) = (27.0 * (L1(f, fs, k) - L1(f, fs, k - 1)) - (L1(f, fs, k + 1) - L1(f, fs, k - 2))) / 24.0;
+    if (g_no_sat) return;
+    static const double R[3] = {15.0 / 8, -5.0 / 4, 3.0 / 8};
+    const double c = 0.5 / (7.0 / 18);
+    double sl = 0, sr = 0;
+    for (int r = 0; r < 3; r++) {
+        sl += R[r] * (L1(f, fs, 1 + r) - L1(f, fs, -r));
+        sr += R[r] * (L1(f, fs, nx - 1 - r) - L1(f, fs, nx + r));
+    }
+    L1(o, os, 1) += c * sl;
+    L1(o, os, nx) -= c * sr;
+}
+
+/* interfaces -> centres interpolation, n centres out, reads f[1..n+1]
+ * sbp_interp_i2c_21_mod.f90 L19-57, sbp_interp_i2c_42_mod.f90 L18-198 */
+static void i_i2c(int order, const double *f, ptrdiff_t fs, int n, double *o, ptrdiff_t os)
+{
+    if (order == 21) {
+        for (int k = 1; k <= n; k++) L1(o, os, k) = 0.5 * (L1(f, fs, k + 1) + L1(f, fs, k));
+        return;
+    }
+    const double c13 = C13, c14 = C14;
+    const double Q[3][5] = {
+        {0.5 + c13 + 2 * c14, 0.5 - 2 * c13 - 3 * c14, c13, c14, 0},
+        {-8.0 / 63 - 52.0 / 21 * c13 - 104.0 / 21 * c14, 29.0 / 42 + 104.0 / 21 * c13 + 52.0 / 7 * c14,
+         -52.0 / 21 * c13 + 0.5, -52.0 / 21 * c14 - 4.0 / 63, 0},
+        {26.0 / 25 * c13 + 52.0 / 25 * c14 - 1.0 / 25, -1.0 / 50 - 52.0 / 25 * c13 - 78.0 / 25 * c14,
+         3.0 / 5 + 26.0 / 25 * c13, 13.0 / 25 + 26.0 / 25 * c14, -3.0 / 50}};
+    for (int r = 0; r < 3; r++) {
+        double sl = 0, sr = 0;
+        for (int k = 0; k < 5; k++) {
+            sl += Q[r][k] * L1(f, fs, 1 + k);
+            sr += Q[r][k] * L1(f, fs, n + 1 - k);
+        }
+        L1(o, os, 1 + r) = sl;
+        L1(o, os, n - r) = sr;
+    }
+    for (int k = 4; k <= n - 3; k++)
+        L1(o, os, k) = (9.0 * (L1(f, fs, k + 1) + L1(f, fs, k)) - (L1(f, fs, k + 2) + L1(f, fs, k - 1))) / 16.0;
+}
+
+/* centres -> interfaces interpolation, n centres in, n+1 out
+ * sbp_interp_c2i_21_mod.f90 L19-87, sbp_interp_c2i_42_mod.f90 L13-218 */
+static void i_c2i(int order, const double *f, ptrdiff_t fs, int n, double *o, ptrdiff_t os)
+{
+    int nx = n + 1;
+    if (order == 21) {
+        L1(o, os, 1) = L1(f, fs, 1);
+        for (int k = 2; k <= nx - 1; k++) L1(o, os, k) = 0.5 * (L1(f, fs, k) + L1(f, fs, k - 1));
+        L1(o, os, nx) = L1(f, fs, nx - 1);
+        return;
+    }
+    const double c13 = C13, c14 = C14;
+    const double Q[4][5] = {
+        {39.0 / 28 + 39.0 / 14 * c13 + 39.0 / 7 * c14, -2.0 / 7 - 39.0 / 7 * c13 - 78.0 / 7 * c14,
+         39.0 / 14 * c13 + 39.0 / 7 * c14 - 3.0 / 28, 0, 0},
+        {13.0 / 27 - 52.0 / 27 * c13 - 26.0 / 9 * c14, 29.0 / 54 + 104.0 / 27 * c13 + 52.0 / 9 * c14,
+         -1.0 / 54 - 52.0 / 27 * c13 - 26.0 / 9 * c14, 0, 0},
+        {13.0 / 12 * c13, -13.0 / 6 * c13 + 7.0 / 16, 5.0 / 8 + 13.0 / 12 * c13, -1.0 / 16, 0},
+        {78.0 / 71 * c14, -156.0 / 71 * c14 - 4.0 / 71, 39.0 / 71 + 78.0 / 71 * c14, 81.0 / 142, -9.0 / 142}};
+    for (int r = 0; r < 4; r++) {
+        double sl = 0, sr = 0;
+        for (int k = 0; k < 5; k++) {
+            sl += Q[r][k] * L1(f, fs, 1 + k);
+            sr += Q[r][k] * L1(f, fs, nx - 1 - k);
+        }
+        L1(o, os, 1 + r) = sl;
+        L1(o, os, nx - r) = sr;
+    }
+    for (int k = 5; k <= nx - 4; k++)
+        L1(o, os, k) = (9.0 * (L1(f, fs, k) + L1(f, fs, k - 1)) - (L1(f, fs, k + 1) + L1(f, fs, k - 2))) / 16.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* geometry: metric/metric_2d_ecs_mod.f90 set_curvilinear_mesh L338-411 */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double *J, *Q11, *Q12, *Q22, *rx, *ry, *rz, *w; /* packed interior */
+} geom;
+
+struct swo {
+    int N, order;
+    double a, g, H0, omega, axis[3];
+    int cor_const;
+    double hx;
+    mtype mt[3];
+    geom G[3];
+    double *fJ2;
+    double *qb, *qt, *ql, *qr;
+    hmap m_xy, m_xyv, m_ch;
+    fld h, u, v, gx, gy, ut, vt, w1, w2, up, vp, Ju, Jv, Dx, Dy, div;
+    /* RK4 registers */
+    double *k[4][3], *y[3];
+};
+
+static const double PI = 3.14159265358979323846;
+
+int swo_npts(const swo *s, int mesh)
+{
+    const mtype t = s->mt[mesh];
+    return 6 * npts_axis(s->N, t.xt) * npts_axis(s->N, t.yt);
+}
+
+static void proto2real(int p, const double v[3], double out[3])
+{
+    for (int k = 0; k < 3; k++) out[k] = T_EX[p][k] * v[0] + T_EY[p][k] * v[1] - T_N[p][k] * v[2];
+}
+
+/* covariant basis a1 = dr/dalpha, a2 = dr/dbeta of r = (X,Y,1)/sigma on the
+ * prototype face, and the dual (contravariant) basis b = Qi a */
+static void ecs_basis(double al, double be, double a1[3], double a2[3], double b1[3], double b2[3])
+{
+    double X = tan(al), Y = tan(be);
+    double s2 = 1 + X * X + Y * Y, s = sqrt(s2), s3 = s2 * s;
+    double cx = 1 + X * X, cy = 1 + Y * Y;
+    a1[0] = cx * cy / s3;
+    a1[1] = -X * Y * cx / s3;
+    a1[2] = -X * cx / s3;
+    a2[0] = -X * Y * cy / s3;
+    a2[1] = cx * cy / s3;
+    a2[2] = -Y * cy / s3;
+    double q11 = s2 / cx, q12 = s2 * X * Y / (cx * cy), q22 = s2 / cy;
+    for (int k = 0; k < 3; k++) {
+        b1[k] = q11 * a1[k] + q12 * a2[k];
+        b2[k] = q12 * a1[k] + q22 * a2[k];
+    }
+}
+
+static void build_geometry(swo *s)
+{
+    int N = s->N;
+    /* quadrature_factory_mod.f90 L89-188 and sbp_operators_collection_mod.f90 L169-197 */
+    double Ai[5], Ac[5];
+    int ni, nc;
+    if (s->order == 21) {
+        ni = 2; nc = 2;
+        Ai[0] = 0.5; Ai[1] = 1;
+        Ac[0] = 1; Ac[1] = 1;
+    } else {
+        ni = 5; nc = 4;
+        Ai[0] = 7.0 / 18; Ai[1] = 9.0 / 8; Ai[2] = 1; Ai[3] = 71.0 / 72; Ai[4] = 1;
+        Ac[0] = 13.0 / 12; Ac[1] = 7.0 / 8; Ac[2] = 25.0 / 24; Ac[3] = 1;
+    }
+    for (int m = 0; m < 3; m++) {
+        mtype t = s->mt[m];
+        int nx = npts_axis(N, t.xt), ny = npts_axis(N, t.yt);
+        size_t n = (size_t)6 * nx * ny;
+        geom *G = &s->G[m];
+        G->J = malloc(n * sizeof(double));
+        G->Q11 = malloc(n * sizeof(double));
+        G->Q12 = malloc(n * sizeof(double));
+        G->Q22 = malloc(n * sizeof(double));
+        G->rx = malloc(n * sizeof(double));
+        G->ry = malloc(n * sizeof(double));
+        G->rz = malloc(n * sizeof(double));
+        G->w = malloc(n * sizeof(double));
+        double shx = t.xt ? 0.5 : 0.0, shy = t.yt ? 0.5 : 0.0;
+        for (int p = 0; p < 6; p++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) {
+                    size_t id = ((size_t)p * ny + (j - 1)) * nx + (i - 1);
+                    double al = -0.25 * PI + (i - 1 + shx) * s->hx;
+                    double be = -0.25 * PI + (j - 1 + shy) * s->hx;
+                    double X = tan(al), Y = tan(be);
+                    double sg = sqrt(1 + X * X + Y * Y);
+                    double pr[3] = {X / sg, Y / sg, 1 / sg}, r[3];
+                    proto2real(p, pr, r);
+                    G->rx[id] = r[0];
+                    G->ry[id] = r[1];
+                    G->rz[id] = r[2];
+                    G->J[id] = (1 + X * X) * (1 + Y * Y) / (sg * sg * sg);
+                    G->Q11[id] = sg * sg / (1 + X * X);
+                    G->Q12[id] = sg * sg * X * Y / ((1 + X * X) * (1 + Y * Y));
+                    G->Q22[id] = sg * sg / (1 + Y * Y);
+                    /* sbp_quadrature_mod.f90 L22-41: J*hx*hy*Ax(i)*Ay(j) */
+                    const double *Ax = t.xt ? Ac : Ai, *Ay = t.yt ? Ac : Ai;
+                    int nex = t.xt ? nc : ni, ney = t.yt ? nc : ni;
+                    double wx = 1, wy = 1;
+                    if (i <= nex) wx = Ax[i - 1];
+                    if (nx - i + 1 <= nex) wx = Ax[nx - i];
+                    if (j <= ney) wy = Ay[j - 1];
+                    if (ny - j + 1 <= ney) wy = Ay[ny - j];
+                    G->w[id] = G->J[id] * s->hx * s->hx * wx * wy;
+                }
+    }
+    /* coriolis_factory_mod.f90 calc_coriolis_parameter L239-280 and the
+     * f*J*J product of create_coriolis_Ch_sbp L188-194 */
+    int nxy = swo_npts(s, SWO_MESH_XY);
+    s->fJ2 = malloc(nxy * sizeof(double));
+    double an = sqrt(s->axis[0] * s->axis[0] + s->axis[1] * s->axis[1] + s->axis[2] * s->axis[2]);
+    (void)an;
+    for (int id = 0; id < nxy; id++) {
+        const geom *G = &s->G[SWO_MESH_XY];
+        double f;
+        if (s->cor_const) {
+            f = s->omega;
+        } else {
+            double r2 = G->rx[id] * G->rx[id] + G->ry[id] * G->ry[id] + G->rz[id] * G->rz[id];
+            double pr = (s->axis[0] * G->rx[id] + s->axis[1] * G->ry[id] + s->axis[2] * G->rz[id]) / sqrt(r2);
+            f = 2 * s->omega * pr;
+        }
+        s->fJ2[id] = f * G->J[id] * G->J[id];
+    }
+    /* covariant edge-sync weights: ecs_Ah_vec_sync_factory_mod.f90 L97-152.
+     * Each weight is (own covariant vector across the edge) . (neighbour's
+     * contravariant along-edge vector) with the neighbour's prototype face
+     * folded onto this one (a 90 degree rotation about the shared edge). */
+    s->qb = malloc((N + 1) * sizeof(double));
+    s->qt = malloc((N + 1) * sizeof(double));
+    s->ql = malloc((N + 1) * sizeof(double));
+    s->qr = malloc((N + 1) * sizeof(double));
+    for (int i = 1; i <= N + 1; i++) {
+        double c = -0.25 * PI + (i - 1) * s->hx, e = 0.25 * PI;
+        double a1[3], a2[3], b1[3], b2[3], A1[3], A2[3], B1[3], B2[3], v[3];
+        /* bottom: own a2 at beta=-pi/4; neighbour b1 at its top edge */
+        ecs_basis(c, -e, a1, a2, b1, b2);
+        ecs_basis(c, e, A1, A2, B1, B2);
+        v[0] = B1[0]; v[1] = -B1[2]; v[2] = B1[1];
+        s->qb[i - 1] = a2[0] * v[0] + a2[1] * v[1] + a2[2] * v[2];
+        /* top */
+        ecs_basis(c, e, a1, a2, b1, b2);
+        ecs_basis(c, -e, A1, A2, B1, B2);
+        v[0] = B1[0]; v[1] = B1[2]; v[2] = -B1[1];
+        s->qt[i - 1] = a2[0] * v[0] + a2[1] * v[1] + a2[2] * v[2];
+        /* left: own a1 at alpha=-pi/4; neighbour b2 at its right edge */
+        ecs_basis(-e, c, a1, a2, b1, b2);
+        ecs_basis(e, c, A1, A2, B1, B2);
+        v[0] = -B2[2]; v[1] = B2[1]; v[2] = B2[0];
+        s->ql[i - 1] = a1[0] * v[0] + a1[1] * v[1] + a1[2] * v[2];
+        /* right */
+        ecs_basis(e, c, a1, a2, b1, b2);
+        ecs_basis(-e, c, A1, A2, B1, B2);
+        v[0] = B2[2]; v[1] = B2[1]; v[2] = -B2[0];
+        s->qr[i - 1] = a1[0] * v[0] + a1[1] * v[1] + a1[2] * v[2];
+    }
+}
+
+swo *swo_create(int N, int order, double radius, double grav, double h_mean,
+                double omega, const double axis[3], int coriolis_const)
+{
+    if (!(order == 21 || order == 42)) return NULL;
+    if (order == 21 && N < 4) return NULL;
+    if (order == 42 && N < 8) return NULL;
+    swo *s = calloc(1, sizeof(swo));
+    s->N = N;
+    s->order = order;
+    s->a = radius;
+    s->g = grav;
+    s->H0 = h_mean;
+    s->omega = omega;
+    for (int k = 0; k < 3; k++) s->axis[k] = axis[k];
+    s->cor_const = coriolis_const;
+    /* mesh_factory_mod.f90 L36-37: hx = (pi/2) / (nx - 1 + 2 shift) = pi/(2N) */
+    s->hx = 0.5 * PI / N;
+    s->mt[SWO_MESH_XY] = MT_XY;
+    s->mt[SWO_MESH_U] = MT_U;
+    s->mt[SWO_MESH_V] = MT_V;
+    build_geometry(s);
+    build_xy_map(&s->m_xy, N, 1, 0);
+    build_xy_map(&s->m_xyv, N, 1, 1);
+    build_ch_map(&s->m_ch, N, order == 21 ? 2 : 3);
+    fld_init(&s->h, N + 1, N + 1);
+    fld_init(&s->u, N, N + 1);
+    fld_init(&s->v, N + 1, N);
+    fld_init(&s->gx, N, N + 1);
+    fld_init(&s->gy, N + 1, N);
+    fld_init(&s->ut, N, N + 1);
+    fld_init(&s->vt, N + 1, N);
+    fld_init(&s->w1, N + 1, N + 1);
+    fld_init(&s->w2, N + 1, N + 1);
+    fld_init(&s->up, N + 1, N + 1);
+    fld_init(&s->vp, N + 1, N + 1);
+    fld_init(&s->Ju, N, N + 1);
+    fld_init(&s->Jv, N + 1, N);
+    fld_init(&s->Dx, N + 1, N + 1);
+    fld_init(&s->Dy, N + 1, N + 1);
+    fld_init(&s->div, N + 1, N + 1);
+    int n3[3] = {swo_npts(s, SWO_MESH_U), swo_npts(s, SWO_MESH_V), swo_npts(s, SWO_MESH_XY)};
+    for (int c = 0; c < 3; c++) {
+        for (int r = 0; r < 4; r++) s->k[r][c] = malloc(n3[c] * sizeof(double));
+        s->y[c] = malloc(n3[c] * sizeof(double));
+    }
+    return s;
+}
+
+void swo_destroy(swo *s)
+{
+    if (!s) return;
+    for (int m = 0; m < 3; m++) {
+        geom *G = &s->G[m];
+        free(G->J); free(G->Q11); free(G->Q12); free(G->Q22);
+        free(G->rx); free(G->ry); free(G->rz); free(G->w);
+    }
+    free(s->fJ2); free(s->qb); free(s->qt); free(s->ql); free(s->qr);
+    free(s->m_xy.e); free(s->m_xyv.e); free(s->m_ch.e);
+    fld *F[] = {&s->h, &s->u, &s->v, &s->gx, &s->gy, &s->ut, &s->vt, &s->w1, &s->w2,
+                &s->up, &s->vp, &s->Ju, &s->Jv, &s->Dx, &s->Dy, &s->div};
+    for (size_t k = 0; k < sizeof(F) / sizeof(F[0]); k++) fld_free(F[k]);
+    for (int c = 0; c < 3; c++) {
+        for (int r = 0; r < 4; r++) free(s->k[r][c]);
+        free(s->y[c]);
+    }
+    free(s);
+}
+
+/* packed index of (p,i,j) on mesh m */
+static inline size_t PK(const swo *s, int m, int p, int i, int j)
+{
+    int nx = npts_axis(s->N, s->mt[m].xt), ny = npts_axis(s->N, s->mt[m].yt);
+    return ((size_t)p * ny + (j - 1)) * nx + (i - 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* operators (all act on the halo-padded work fields inside swo)       */
+/* ------------------------------------------------------------------ */
+
+/* grad_sbp_sat_mod.f90 L24-76: h (xy) -> gx (mesh y), gy (mesh x) */
+static void op_grad(swo *s)
+{
+    int N = s->N;
+    if (g_mutation != 3) apply_map(&s->m_xy, &s->h, &s->h);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N + 1; j++) {
+            d_i2c(s->order, AT(&s->h, p, 0, j), 1, N, AT(&s->gx, p, 0, j), 1);
+            for (int i = 1; i <= N; i++) *AT(&s->gx, p, i, j) /= (s->hx * s->a);
+        }
+        for (int i = 1; i <= N + 1; i++) {
+            d_i2c(s->order, AT(&s->h, p, i, 0), s->h.sx, N, AT(&s->gy, p, i, 0), s->gy.sx);
+            for (int j = 1; j <= N; j++) *AT(&s->gy, p, i, j) /= (s->hx * s->a);
+        }
+    }
+}
+
+/* co2contra_Ch_mod.f90 L27-94: covariant (u,v) -> contravariant (ut,vt) */
+static void op_co2contra(swo *s)
+{
+    int N = s->N;
+    const geom *Gp = &s->G[SWO_MESH_XY], *Gu = &s->G[SWO_MESH_U], *Gv = &s->G[SWO_MESH_V];
+    for (int p = 0; p < 6; p++) {
+        /* uv -> xy (c2i): uq along x, vq along y */
+        for (int j = 1; j <= N + 1; j++)
+            i_c2i(s->order, AT(&s->u, p, 0, j), 1, N, AT(&s->w2, p, 0, j), 1);
+        for (int i = 1; i <= N + 1; i++)
+            i_c2i(s->order, AT(&s->v, p, i, 0), s->v.sx, N, AT(&s->w1, p, i, 0), s->w1.sx);
+        /* at xy: w1 <- vq*J*Qi12 (feeds ut), w2 <- uq*J*Qi12 (feeds vt) */
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N + 1; i++) {
+                size_t id = PK(s, SWO_MESH_XY, p, i, j);
+                double c = Gp->J[id] * Gp->Q12[id];
+                *AT(&s->w1, p, i, j) *= c;
+                *AT(&s->w2, p, i, j) *= c;
+            }
+        /* xy -> uv (i2c) and finalize */
+        for (int j = 1; j <= N + 1; j++)
+            i_i2c(s->order, AT(&s->w1, p, 0, j), 1, N, AT(&s->ut, p, 0, j), 1);
+        for (int i = 1; i <= N + 1; i++)
+            i_i2c(s->order, AT(&s->w2, p, i, 0), s->w2.sx, N, AT(&s->vt, p, i, 0), s->vt.sx);
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++) {
+                size_t id = PK(s, SWO_MESH_U, p, i, j);
+                *AT(&s->ut, p, i, j) = Gu->Q11[id] * *AT(&s->u, p, i, j) + *AT(&s->ut, p, i, j) / Gu->J[id];
+            }
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++) {
+                size_t id = PK(s, SWO_MESH_V, p, i, j);
+                *AT(&s->vt, p, i, j) = Gv->Q22[id] * *AT(&s->v, p, i, j) + *AT(&s->vt, p, i, j) / Gv->J[id];
+            }
+    }
+}
+
+/* covariant vector sync at xy edge points, ecs_Ah_vec_sync_mod.f90 L34-125
+ * called with (v,u) swapped for covariant components (L51-53).  Reads the
+ * neighbour's components from the depth-1 halo (the duplicate points). */
+static void sync_covariant(swo *s, fld *up, fld *vp)
+{
+    int N = s->N, e = N + 1;
+    const double *qb = s->qb, *qt = s->qt, *ql = s->ql, *qr = s->qr;
+    double *zero = NULL;
+    if (g_mutation == 2) {
+        zero = calloc(N + 1, sizeof(double));
+        qb = qt = ql = qr = zero;
+    }
+    for (int p = 0; p < 6; p++) {
+#define U(i, j) (*AT(vp, p, i, j)) /* routine's first component  */
+#define V(i, j) (*AT(up, p, i, j)) /* routine's second component */
+        double u11 = (U(1, 1) + U(0, 1) + U(1, 0) + qb[0] * V(1, 0)) / 3.0;
+        double v11 = (V(1, 1) + V(0, 1) + V(1, 0) + ql[0] * U(0, 1)) / 3.0;
+        double ue1 = (U(e, 1) + U(e + 1, 1) + U(e, 0) + qb[e - 1] * V(e, 0)) / 3.0;
+        double ve1 = (V(e, 1) + V(e + 1, 1) + V(e, 0) + qr[0] * U(e + 1, 1)) / 3.0;
+        double u1e = (U(1, e) + U(0, e) + U(1, e + 1) + qt[0] * V(1, e + 1)) / 3.0;
+        double v1e = (V(1, e) + V(0, e) + V(1, e + 1) + ql[e - 1] * U(0, e)) / 3.0;
+        double uee = (U(e, e) + U(e + 1, e) + U(e, e + 1) + qt[e - 1] * V(e, e + 1)) / 3.0;
+        double vee = (V(e, e) + V(e + 1, e) + V(e, e + 1) + qr[e - 1] * U(e + 1, e)) / 3.0;
+        U(1, 1) = u11; V(1, 1) = v11;
+        U(e, 1) = ue1; V(e, 1) = ve1;
+        U(1, e) = u1e; V(1, e) = v1e;
+        U(e, e) = uee; V(e, e) = vee;
+        for (int j = 2; j <= N; j++) {
+            U(1, j) = 0.5 * (U(0, j) + U(1, j));
+            V(1, j) = 0.5 * (V(1, j) + V(0, j) + ql[j - 1] * U(0, j));
+            U(e, j) = 0.5 * (U(e, j) + U(e + 1, j));
+            V(e, j) = 0.5 * (V(e, j) + V(e + 1, j) + qr[j - 1] * U(e + 1, j));
+        }
+        for (int i = 2; i <= N; i++) {
+            V(i, 1) = 0.5 * (V(i, 0) + V(i, 1));
+            U(i, 1) = 0.5 * (U(i, 0) + U(i, 1) + qb[i - 1] * V(i, 0));
+            V(i, e) = 0.5 * (V(i, e) + V(i, e + 1));
+            U(i, e) = 0.5 * (U(i, e) + U(i, e + 1) + qt[i - 1] * V(i, e + 1));
+        }
+#undef U
+#undef V
+    }
+    free(zero);
+}
+
+/* coriolis_Ch_mod.f90 calc_coriolis L42-69: (ut,vt) -> (cu on y, cv on x) */
+static void op_coriolis(swo *s, fld *cu, fld *cv)
+{
+    int N = s->N;
+    const geom *Gu = &s->G[SWO_MESH_U], *Gv = &s->G[SWO_MESH_V];
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N + 1; j++)
+            i_c2i(s->order, AT(&s->ut, p, 0, j), 1, N, AT(&s->w1, p, 0, j), 1);
+        for (int i = 1; i <= N + 1; i++)
+            i_c2i(s->order, AT(&s->vt, p, i, 0), s->vt.sx, N, AT(&s->w2, p, i, 0), s->w2.sx);
+        /* calc_cori_at_p_tile L97-120: (up,vp) <- fJ2*(vp,-up) */
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N + 1; i++) {
+                double f = s->fJ2[PK(s, SWO_MESH_XY, p, i, j)];
+                *AT(&s->up, p, i, j) = f * *AT(&s->w2, p, i, j);
+                *AT(&s->vp, p, i, j) = -f * *AT(&s->w1, p, i, j);
+            }
+    }
+    apply_map(&s->m_xyv, &s->up, &s->vp);
+    sync_covariant(s, &s->up, &s->vp);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N + 1; j++) {
+            i_i2c(s->order, AT(&s->up, p, 0, j), 1, N, AT(cu, p, 0, j), 1);
+            for (int i = 1; i <= N; i++) *AT(cu, p, i, j) /= Gu->J[PK(s, SWO_MESH_U, p, i, j)];
+        }
+        for (int i = 1; i <= N + 1; i++) {
+            i_i2c(s->order, AT(&s->vp, p, i, 0), s->vp.sx, N, AT(cv, p, i, 0), cv->sx);
+            for (int j = 1; j <= N; j++) *AT(cv, p, i, j) /= Gv->J[PK(s, SWO_MESH_V, p, i, j)];
+        }
+    }
+}
+
+/* halo_Ah_scalar_sync_mod.f90 L43-98: average the duplicated edge values */
+static void sync_scalar(swo *s, fld *f)
+{
+    int N = s->N, e = N + 1;
+    apply_map(&s->m_xy, f, f);
+    for (int p = 0; p < 6; p++) {
+#define F(i, j) (*AT(f, p, i, j))
+        double c11 = (F(1, 1) + F(0, 1) + F(1, 0)) / 3.0;
+        double ce1 = (F(e, 1) + F(e + 1, 1) + F(e, 0)) / 3.0;
+        double c1e = (F(1, e) + F(0, e) + F(1, e + 1)) / 3.0;
+        double cee = (F(e, e) + F(e + 1, e) + F(e, e + 1)) / 3.0;
+        F(1, 1) = c11; F(e, 1) = ce1; F(1, e) = c1e; F(e, e) = cee;
+        for (int j = 2; j <= N; j++) {
+            F(1, j) = 0.5 * (F(0, j) + F(1, j));
+            F(e, j) = 0.5 * (F(e, j) + F(e + 1, j));
+        }
+        for (int i = 2; i <= N; i++) {
+            F(i, 1) = 0.5 * (F(i, 1) + F(i, 0));
+            F(i, e) = 0.5 * (F(i, e) + F(i, e + 1));
+        }
+#undef F
+    }
+}
+
+/* div_sbp_sat_mod.f90 L29-80 + projection: (ut,vt) -> div (xy) */
+static void op_div(swo *s)
+{
+    int N = s->N;
+    const geom *Gp = &s->G[SWO_MESH_XY], *Gu = &s->G[SWO_MESH_U], *Gv = &s->G[SWO_MESH_V];
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++)
+                *AT(&s->Ju, p, i, j) = *AT(&s->ut, p, i, j) * Gu->J[PK(s, SWO_MESH_U, p, i, j)];
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++)
+                *AT(&s->Jv, p, i, j) = *AT(&s->vt, p, i, j) * Gv->J[PK(s, SWO_MESH_V, p, i, j)];
+    }
+    apply_map(&s->m_ch, &s->Ju, &s->Jv);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N + 1; j++)
+            d_c2i(s->order, AT(&s->Ju, p, 0, j), 1, N, AT(&s->Dx, p, 0, j), 1);
+        for (int i = 1; i <= N + 1; i++)
+            d_c2i(s->order, AT(&s->Jv, p, i, 0), s->Jv.sx, N, AT(&s->Dy, p, i, 0), s->Dy.sx);
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N + 1; i++)
+                *AT(&s->div, p, i, j) = (*AT(&s->Dx, p, i, j) / s->hx + *AT(&s->Dy, p, i, j) / s->hx) /
+                                        (Gp->J[PK(s, SWO_MESH_XY, p, i, j)] * s->a);
+    }
+    if (g_mutation != 5) sync_scalar(s, &s->div);
+}
+
+/* operator_swm_lin_mod.f90 apply L66-108 (fields already loaded) */
+static void apply_loaded(swo *s, double *du, double *dv, double *dh)
+{
+    int N = s->N;
+    op_grad(s);
+    op_co2contra(s);
+    /* coriolis writes into the ut/vt-shaped fields cu=Dx-like work: reuse Ju/Jv
+     * after div needs them, so use dedicated outputs here */
+    fld cu, cv;
+    fld_init(&cu, N, N + 1);
+    fld_init(&cv, N + 1, N);
+    op_coriolis(s, &cu, &cv);
+    op_div(s);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++)
+                du[PK(s, SWO_MESH_U, p, i, j)] = -s->g * *AT(&s->gx, p, i, j) + *AT(&cu, p, i, j);
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++)
+                dv[PK(s, SWO_MESH_V, p, i, j)] = -s->g * *AT(&s->gy, p, i, j) + *AT(&cv, p, i, j);
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N + 1; i++)
+                dh[PK(s, SWO_MESH_XY, p, i, j)] = -s->H0 * *AT(&s->div, p, i, j);
+    }
+    fld_free(&cu);
+    fld_free(&cv);
+}
+
+void swo_apply(swo *s, const double *u, const double *v, const double *h,
+               double *du, double *dv, double *dh)
+{
+    fld_load(&s->u, u);
+    fld_load(&s->v, v);
+    fld_load(&s->h, h);
+    apply_loaded(s, du, dv, dh);
+}
+
+void swo_grad(swo *s, const double *h, double *gx, double *gy)
+{
+    fld_load(&s->h, h);
+    op_grad(s);
+    fld_store(&s->gx, gx);
+    fld_store(&s->gy, gy);
+}
+
+void swo_co2contra(swo *s, const double *u, const double *v, double *ut, double *vt)
+{
+    fld_load(&s->u, u);
+    fld_load(&s->v, v);
+    op_co2contra(s);
+    fld_store(&s->ut, ut);
+    fld_store(&s->vt, vt);
+}
+
+void swo_coriolis(swo *s, const double *ut, const double *vt, double *cu, double *cv)
+{
+    fld_load(&s->ut, ut);
+    fld_load(&s->vt, vt);
+    fld a, b;
+    fld_init(&a, s->N, s->N + 1);
+    fld_init(&b, s->N + 1, s->N);
+    op_coriolis(s, &a, &b);
+    fld_store(&a, cu);
+    fld_store(&b, cv);
+    fld_free(&a);
+    fld_free(&b);
+}
+
+void swo_div(swo *s, const double *ut, const double *vt, double *div)
+{
+    fld_load(&s->ut, ut);
+    fld_load(&s->vt, vt);
+    op_div(s);
+    fld_store(&s->div, div);
+}
+
+/* rk4_mod.f90 step_rk4 L20-42, with stvec assign/update of
+ * grid_field_mod.f90 L460-534 (interior points only) */
+void swo_rk4(swo *s, double *u, double *v, double *h, double dt, int nsteps)
+{
+    int n3[3] = {swo_npts(s, SWO_MESH_U), swo_npts(s, SWO_MESH_V), swo_npts(s, SWO_MESH_XY)};
+    double *v0[3] = {u, v, h};
+    for (int it = 0; it < nsteps; it++) {
+        swo_apply(s, v0[0], v0[1], v0[2], s->k[0][0], s->k[0][1], s->k[0][2]);
+        for (int c = 0; c < 3; c++)
+            for (int n = 0; n < n3[c]; n++) s->y[c][n] = 1.0 * v0[c][n] + 0.5 * dt * s->k[0][c][n];
+        swo_apply(s, s->y[0], s->y[1], s->y[2], s->k[1][0], s->k[1][1], s->k[1][2]);
+        for (int c = 0; c < 3; c++)
+            for (int n = 0; n < n3[c]; n++) s->y[c][n] = 1.0 * v0[c][n] + 0.5 * dt * s->k[1][c][n];
+        swo_apply(s, s->y[0], s->y[1], s->y[2], s->k[2][0], s->k[2][1], s->k[2][2]);
+        for (int c = 0; c < 3; c++)
+            for (int n = 0; n < n3[c]; n++) s->y[c][n] = 1.0 * v0[c][n] + dt * s->k[2][c][n];
+        swo_apply(s, s->y[0], s->y[1], s->y[2], s->k[3][0], s->k[3][1], s->k[3][2]);
+        for (int c = 0; c < 3; c++)
+            for (int n = 0; n < n3[c]; n++) {
+                v0[c][n] = v0[c][n] + dt / 6.0 * s->k[0][c][n] + dt / 3.0 * s->k[1][c][n];
+                v0[c][n] = v0[c][n] + dt / 3.0 * s->k[2][c][n] + dt / 6.0 * s->k[3][c][n];
+            }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* geometry export, integrals, test fields                             */
+/* ------------------------------------------------------------------ */
+void swo_metric(const swo *s, int mesh, double *J, double *Qi11, double *Qi12,
+                double *Qi22, double *rx, double *ry, double *rz)
+{
+    int n = swo_npts(s, mesh);
+    const geom *G = &s->G[mesh];
+    size_t b = n * sizeof(double);
+    if (J) memcpy(J, G->J, b);
+    if (Qi11) memcpy(Qi11, G->Q11, b);
+    if (Qi12) memcpy(Qi12, G->Q12, b);
+    if (Qi22) memcpy(Qi22, G->Q22, b);
+    if (rx) memcpy(rx, G->rx, b);
+    if (ry) memcpy(ry, G->ry, b);
+    if (rz) memcpy(rz, G->rz, b);
+}
+
+void swo_quad_weights(const swo *s, int mesh, double *w)
+{
+    memcpy(w, s->G[mesh].w, swo_npts(s, mesh) * sizeof(double));
+}
+
+void swo_coriolis_fJ2(const swo *s, double *fJ2)
+{
+    memcpy(fJ2, s->fJ2, swo_npts(s, SWO_MESH_XY) * sizeof(double));
+}
+
+void swo_sync_weights(const swo *s, double *qb, double *qt, double *ql, double *qr)
+{
+    size_t b = (s->N + 1) * sizeof(double);
+    memcpy(qb, s->qb, b);
+    memcpy(qt, s->qt, b);
+    memcpy(ql, s->ql, b);
+    memcpy(qr, s->qr, b);
+}
+
+double swo_mass(const swo *s, const double *h)
+{
+    double m = 0;
+    int n = swo_npts(s, SWO_MESH_XY);
+    for (int k = 0; k < n; k++) m += h[k] * s->G[SWO_MESH_XY].w[k];
+    return m;
+}
+
+/* Linear SBP energy E = g/2 <h,h>_xy + H0/2 (<u,ut>_y + <v,vt>_x).  The
+ * reference keeps its own linear-energy code commented out
+ * (operator_swm_lin_mod.f90 L244-269); this is the quadratic form the SBP
+ * discretisation conserves. */
+double swo_energy(swo *s, const double *u, const double *v, const double *h)
+{
+    int nu = swo_npts(s, SWO_MESH_U), nv = swo_npts(s, SWO_MESH_V), nh = swo_npts(s, SWO_MESH_XY);
+    double *ut = malloc(nu * sizeof(double)), *vt = malloc(nv * sizeof(double));
+    swo_co2contra(s, u, v, ut, vt);
+    double eh = 0, eu = 0, ev = 0;
+    for (int k = 0; k < nh; k++) eh += h[k] * h[k] * s->G[SWO_MESH_XY].w[k];
+    for (int k = 0; k < nu; k++) eu += u[k] * ut[k] * s->G[SWO_MESH_U].w[k];
+    for (int k = 0; k < nv; k++) ev += v[k] * vt[k] * s->G[SWO_MESH_V].w[k];
+    free(ut);
+    free(vt);
+    return 0.5 * s->g * eh + 0.5 * s->H0 * (eu + ev);
+}
+
+/* test_fields_mod.f90 generate_gaussian_hill_scalar_field L491-514 (times amp) */
+void swo_gaussian_hill(const swo *s, double lam0, double phi0, double d0,
+                       double amp, double *h)
+{
+    const geom *G = &s->G[SWO_MESH_XY];
+    double x0 = cos(phi0) * cos(lam0), y0 = cos(phi0) * sin(lam0), z0 = sin(phi0);
+    int n = swo_npts(s, SWO_MESH_XY);
+    for (int k = 0; k < n; k++) {
+        double c = G->rx[k] * x0 + G->ry[k] * y0 + G->rz[k] * z0;
+        c = c > 1 ? 1 : (c < -1 ? -1 : c);
+        double r = acos(c);
+        h[k] = amp * exp(-(r / d0) * (r / d0));
+    }
+}
+
+/* covariant component of a Cartesian vector field: V . a1 (mesh u) or
+ * V . a2 (mesh v); test_fields_mod.f90 set_vector_test_field L292-329 */
+void swo_covariant_from_cart(const swo *s, const double *vx, const double *vy,
+                             const double *vz, int mesh, double *out)
+{
+    int N = s->N;
+    mtype t = s->mt[mesh];
+    int nx = npts_axis(N, t.xt), ny = npts_axis(N, t.yt);
+    double shx = t.xt ? 0.5 : 0.0, shy = t.yt ? 0.5 : 0.0;
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                size_t id = ((size_t)p * ny + (j - 1)) * nx + (i - 1);
+                double al = -0.25 * PI + (i - 1 + shx) * s->hx;
+                double be = -0.25 * PI + (j - 1 + shy) * s->hx;
+                double a1[3], a2[3], b1[3], b2[3], A[3];
+                ecs_basis(al, be, a1, a2, b1, b2);
+                proto2real(p, mesh == SWO_MESH_U ? a1 : a2, A);
+                out[id] = vx[id] * A[0] + vy[id] * A[1] + vz[id] * A[2];
+            }
+}
+
+/* Williamson test 2 in linear mode: Williamson_test2_mod.f90 L32-112 with
+ * test_fields_mod.f90 generate_ts2_height_field L857-894 (metric_term = 0)
+ * and gen_solid_rotation_vec_field L586-612 (V = u0 axis x r). */
+void swo_ts2_linear(const swo *s, double u0, double h_mean, double *u,
+                    double *v, double *h)
+{
+    double an = sqrt(s->axis[0] * s->axis[0] + s->axis[1] * s->axis[1] + s->axis[2] * s->axis[2]);
+    double ax = s->axis[0] / an, ay = s->axis[1] / an, az = s->axis[2] / an;
+    const geom *G = &s->G[SWO_MESH_XY];
+    int nh = swo_npts(s, SWO_MESH_XY);
+    for (int k = 0; k < nh; k++) {
+        double r = sqrt(G->rx[k] * G->rx[k] + G->ry[k] * G->ry[k] + G->rz[k] * G->rz[k]);
+        double sp = (G->rx[k] * ax + G->ry[k] * ay + G->rz[k] * az) / r;
+        double phi = asin(sp);
+        h[k] = h_mean - (s->a * s->omega * u0) / s->g * sin(phi) * sin(phi);
+    }
+    for (int m = SWO_MESH_U; m <= SWO_MESH_V; m++) {
+        const geom *Gm = &s->G[m];
+        int n = swo_npts(s, m);
+        double *vx = malloc(n * sizeof(double)), *vy = malloc(n * sizeof(double)), *vz = malloc(n * sizeof(double));
+        for (int k = 0; k < n; k++) {
+            double r = sqrt(Gm->rx[k] * Gm->rx[k] + Gm->ry[k] * Gm->ry[k] + Gm->rz[k] * Gm->rz[k]);
+            double xn = Gm->rx[k] / r, yn = Gm->ry[k] / r, zn = Gm->rz[k] / r;
+            vx[k] = u0 * (ay * zn - az * yn);
+            vy[k] = -u0 * (ax * zn - az * xn);
+            vz[k] = u0 * (ax * yn - ay * xn);
+        }
+        swo_covariant_from_cart(s, vx, vy, vz, m, m == SWO_MESH_U ? u : v);
+        free(vx); free(vy); free(vz);
+    }
+}
+
+int swo_halo_map(const swo *s, int kind, int *tp, int *ti, int *tj, int *tc,
+                 int *sp, int *si, int *sj, int *sc, int *sign)
+{
+    const hmap *m = kind == 0 ? &s->m_xy : (kind == 1 ? &s->m_ch : &s->m_xyv);
+    if (tp)
+        for (int k = 0; k < m->n; k++) {
+            const hentry *e = &m->e[k];
+            tp[k] = e->tp; ti[k] = e->ti; tj[k] = e->tj; tc[k] = e->tc;
+            sp[k] = e->sp; si[k] = e->si; sj[k] = e->sj; sc[k] = e->sc;
+            sign[k] = e->sign;
+        }
+    return m->n;
+}
+
+/* 1-D operator on one line for identity tests: op 0 d_i2c, 1 d_c2i,
+ * 2 i_i2c, 3 i_c2i.  in points at index 1-HALO of a line padded by HALO on
+ * both sides; n = number of centres.  with_sat = 0 drops the SAT terms. */
+void swo_line_op(int order, int op, int n, const double *in, double *out, int with_sat)
+{
+    double *o = calloc(n + 1 + 2 * HALO, sizeof(double));
+    const double *f = in + HALO - 1; /* index 0 */
+    double *oo = o + HALO - 1;   /* out index 1 at o[HALO] */
+    g_no_sat = !with_sat;
+    switch (op) {
+    case 0: d_i2c(order, f, 1, n, oo, 1); break;
+    case 1: d_c2i(order, f, 1, n, oo, 1); break;
+    case 2: i_i2c(order, f, 1, n, oo, 1); break;
+    default: i_c2i(order, f, 1, n, oo, 1); break;
+    }
+    g_no_sat = 0;
+    int nout = (op == 0 || op == 2) ? n : n + 1;
+    for (int k = 1; k <= nout; k++) out[k - 1] = oo[k];
+    free(o);
+}
+
+/* covariant basis vectors a1, a2 (3-D, unit sphere) at every point of a
+ * mesh, packed as [npts][3] */
+void swo_basis(const swo *s, int mesh, double *a1out, double *a2out)
+{
+    int N = s->N;
+    mtype t = s->mt[mesh];
+    int nx = npts_axis(N, t.xt), ny = npts_axis(N, t.yt);
+    double shx = t.xt ? 0.5 : 0.0, shy = t.yt ? 0.5 : 0.0;
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                size_t id = ((size_t)p * ny + (j - 1)) * nx + (i - 1);
+                double al = -0.25 * PI + (i - 1 + shx) * s->hx;
+                double be = -0.25 * PI + (j - 1 + shy) * s->hx;
+                double a1[3], a2[3], b1[3], b2[3];
+                ecs_basis(al, be, a1, a2, b1, b2);
+                proto2real(p, a1, a1out + 3 * id);
+                proto2real(p, a2, a2out + 3 * id);
+            }
+}
