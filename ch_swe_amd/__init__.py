@@ -1,0 +1,242 @@
+"""ch_swe_amd: MI355X-native linear shallow-water RK4 step on the staggered
+(Ch) equiangular cubed sphere, SBP orders 21 and 42.
+
+Python host mirror of the reference's interface for this path:
+
+    reference (Fortran)                              here
+    create_linear_swm_operator(op, grav, cfg, dom)   create_linear_swm_operator(...)
+      (operator_swm_factory_mod.f90 L586-685)
+    operator_t%apply(vout, vin, domain)              OperatorSwmLin.apply(vout, vin)
+      (operator_mod.f90 L11-41, operator_swm_lin_mod.f90 L66-108)
+    stvec_t%assign / %update / %create_similar       StvecSwm.assign / .update / .create_similar
+      (stvec/stvec_mod.f90 L10-30)
+    create_timescheme(ts, v, "rk4")                  create_timescheme("rk4", op)
+    rk4_t%step(v0, operator, domain, dt)             Rk4.step(v0, operator, dt)
+      (time_schemes/rk4_mod.f90 L20-42)
+
+Every call goes through the C ABI of include/ch_swe.h (libchswe.so, built
+in-tree for gfx950).  There is no CPU fallback: if the library is missing or
+no device is present, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "SweError", "lib", "Config", "OperatorSwmLin", "StvecSwm", "Rk4",
+    "create_linear_swm_operator", "create_timescheme", "halo_map",
+    "FP64", "FP32", "LIB_PATH",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libchswe.so")
+FP64, FP32 = 0, 1
+
+# every entry point declared in include/ch_swe.h
+EXPORTS = (
+    "swe_last_error", "swe_version", "swe_plan_create", "swe_plan_destroy", "swe_plan_npts",
+    "swe_plan_stream", "swe_synchronize", "swe_state_create", "swe_state_destroy",
+    "swe_state_upload", "swe_state_download", "swe_apply", "swe_assign", "swe_update",
+    "swe_rk4_steps", "swe_integrals", "swe_halo_map",
+)
+
+
+class SweError(RuntimeError):
+    """A non-zero return code from the C ABI (message from swe_last_error)."""
+
+
+class _CConfig(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("order", ctypes.c_int), ("precision", ctypes.c_int),
+                ("coriolis_const", ctypes.c_int), ("radius", ctypes.c_double),
+                ("grav", ctypes.c_double), ("h_mean", ctypes.c_double),
+                ("omega", ctypes.c_double), ("axis", ctypes.c_double * 3)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+def lib():
+    """Load libchswe.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with ch_swe_amd/build.py "
+                          "(or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    L.swe_last_error.restype = ctypes.c_char_p
+    L.swe_version.restype = ctypes.c_char_p
+    L.swe_plan_create.argtypes = [ctypes.POINTER(_CConfig), ctypes.POINTER(_vp)]
+    L.swe_plan_destroy.argtypes = [_vp]
+    L.swe_plan_npts.argtypes = [_vp, _ip, _ip, _ip]
+    L.swe_plan_stream.argtypes = [_vp]
+    L.swe_plan_stream.restype = _vp
+    L.swe_synchronize.argtypes = [_vp]
+    L.swe_state_create.argtypes = [_vp, ctypes.POINTER(_vp)]
+    L.swe_state_destroy.argtypes = [_vp]
+    L.swe_state_upload.argtypes = [_vp, _dp, _dp, _dp]
+    L.swe_state_download.argtypes = [_vp, _dp, _dp, _dp]
+    L.swe_apply.argtypes = [_vp, _vp, _vp]
+    L.swe_assign.argtypes = [_vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp]
+    L.swe_update.argtypes = [_vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp]
+    L.swe_rk4_steps.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_int]
+    L.swe_integrals.argtypes = [_vp, _vp, _dp, _dp]
+    L.swe_halo_map.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip] + [_ip] * 9
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().swe_last_error().decode()
+        raise SweError(f"ch_swe error {rc}: {msg}")
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Config:
+    """Operator configuration (the reference's domain/metric/operator config
+    keys: N, planet_radius, omega, rotation_axis, coriolis_constant, h_mean,
+    grav, SBP order via the operator names)."""
+
+    def __init__(self, N, order=42, precision=FP64, radius=6.371e6, grav=9.81,
+                 h_mean=1000.0, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
+                 coriolis_const=False):
+        self.N, self.order, self.precision = int(N), int(order), int(precision)
+        self.radius, self.grav, self.h_mean, self.omega = radius, grav, h_mean, omega
+        self.axis = tuple(float(a) for a in axis)
+        self.coriolis_const = bool(coriolis_const)
+
+    def _c(self):
+        c = _CConfig()
+        c.N, c.order, c.precision = self.N, self.order, self.precision
+        c.coriolis_const = 1 if self.coriolis_const else 0
+        c.radius, c.grav, c.h_mean, c.omega = self.radius, self.grav, self.h_mean, self.omega
+        c.axis[:] = self.axis
+        return c
+
+
+class _Plan:
+    def __init__(self, cfg):
+        L = lib()
+        h = _vp()
+        _check(L.swe_plan_create(ctypes.byref(cfg._c()), ctypes.byref(h)))
+        self.h = h
+        self.cfg = cfg
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().swe_plan_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        self.close()
+
+
+class StvecSwm:
+    """Device-resident state (u, v, h): stvec_swm_t
+    (models/shallow_water/stvec/stvec_swm_mod.f90 L10-42)."""
+
+    def __init__(self, plan):
+        self._plan = plan
+        h = _vp()
+        _check(lib().swe_state_create(plan.h, ctypes.byref(h)))
+        self.h = h
+        N = plan.cfg.N
+        self.shapes = {"u": (6, N + 1, N), "v": (6, N, N + 1), "h": (6, N + 1, N + 1)}
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().swe_state_destroy(self.h)
+            self.h = _vp()
+
+    def create_similar(self):
+        return StvecSwm(self._plan)
+
+    def upload(self, u, v, h):
+        arrs = []
+        for name, a in (("u", u), ("v", v), ("h", h)):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.size != int(np.prod(self.shapes[name])):
+                raise ValueError(f"{name}: expected {self.shapes[name]}, got {a.shape}")
+            arrs.append(a)
+        _check(lib().swe_state_upload(self.h, *[_p(a) for a in arrs]))
+        return self
+
+    def download(self):
+        out = [np.empty(self.shapes[k]) for k in ("u", "v", "h")]
+        _check(lib().swe_state_download(self.h, *[_p(a) for a in out]))
+        return tuple(out)
+
+    # stvec_t%assign (s1*v1 [+ s2*v2]) and %update (+= s1*v1 [+ s2*v2])
+    def assign(self, s1, v1, s2=0.0, v2=None):
+        _check(lib().swe_assign(self._plan.h, self.h, s1, v1.h, s2, v2.h if v2 is not None else None))
+
+    def update(self, s1, v1, s2=0.0, v2=None):
+        _check(lib().swe_update(self._plan.h, self.h, s1, v1.h, s2, v2.h if v2 is not None else None))
+
+
+class OperatorSwmLin:
+    """operator_swm_lin_t: the linear SWE tendency, apply only (explicit
+    schemes never call solve)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.plan = _Plan(cfg)
+
+    def create_stvec(self):
+        return StvecSwm(self.plan)
+
+    def apply(self, vout, vin):
+        _check(lib().swe_apply(self.plan.h, vout.h, vin.h))
+
+    def integrals(self, state):
+        m, e = ctypes.c_double(), ctypes.c_double()
+        _check(lib().swe_integrals(self.plan.h, state.h, ctypes.byref(m), ctypes.byref(e)))
+        return m.value, e.value
+
+    def synchronize(self):
+        _check(lib().swe_synchronize(self.plan.h))
+
+    @property
+    def stream(self):
+        return lib().swe_plan_stream(self.plan.h)
+
+
+class Rk4:
+    """rk4_t (time_schemes/rk4_mod.f90 L12-42)."""
+
+    def step(self, v0, operator, dt, nsteps=1):
+        _check(lib().swe_rk4_steps(operator.plan.h, v0.h, float(dt), int(nsteps)))
+
+
+def create_linear_swm_operator(N, order=42, precision=FP64, grav=9.81, h_mean=1000.0,
+                               radius=6.371e6, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
+                               coriolis_const=False):
+    """Factory mirroring create_linear_swm_operator
+    (operator_swm_factory_mod.f90 L586-685) for the Ch staggering."""
+    return OperatorSwmLin(Config(N, order, precision, radius, grav, h_mean, omega, axis,
+                                 coriolis_const))
+
+
+def create_timescheme(name):
+    """Factory mirroring timescheme_factory_mod.f90 (only 'rk4' is on this path)."""
+    if name != "rk4":
+        raise ValueError(f"unknown time scheme '{name}' (this build provides 'rk4')")
+    return Rk4()
+
+
+def halo_map(N, order, kind):
+    """Host-only edge map of the in-device halo gathers (see swe_halo_map)."""
+    L = lib()
+    n = ctypes.c_int()
+    _check(L.swe_halo_map(N, order, kind, ctypes.byref(n), *([None] * 9)))
+    arrs = [np.zeros(n.value, dtype=np.int32) for _ in range(9)]
+    _check(L.swe_halo_map(N, order, kind, ctypes.byref(n), *[a.ctypes.data_as(_ip) for a in arrs]))
+    return dict(zip(("tp", "ti", "tj", "tc", "sp", "si", "sj", "sc", "sign"), arrs))

@@ -1,0 +1,905 @@
+// libchswe: MI355X (gfx950) implementation of the linear SWE RK4 step on the
+// staggered (Ch) equiangular cubed sphere, behind the C ABI of
+// include/ch_swe.h.
+//
+// Structure of one operator application (operator_swm_lin_mod.f90 L66-108):
+//   gather h duplicates          (grad SAT needs the neighbour's edge value)
+//   k_xy_w      : W1 = J Qi12 Ic2i_y(v), W2 = J Qi12 Ic2i_x(u)        at xy
+//   k_contra    : ut = Qi11 u + Ii2c_x(W1)/J, vt = Qi22 v + Ii2c_y(W2)/J
+//                 (co2contra_Ch_mod.f90 L27-94), plus J ut, J vt
+//   gather J ut / J vt across edges (div SAT normal flux, depth 2|3)
+//   k_xy_cd     : raw Coriolis (up, vp) = fJ2 (Ic2i_y vt, -Ic2i_x ut) and
+//                 raw div = (Dc2i_x(J ut) + Dc2i_y(J vt))/(hx J a)        at xy
+//   gather (up, vp, div) duplicates
+//   k_edge_sync : covariant vector sync of (up, vp) and scalar projection of
+//                 div at the panel boundary (ecs_Ah_vec_sync_mod.f90 L34-125,
+//                 halo_Ah_scalar_sync_mod.f90 L43-98)
+//   k_tend      : du = -g Di2c_x(h)/(hx a) + Ii2c_x(up)/J, dv likewise,
+//                 dh = -H0 div
+// The MPI halo exchanges of the reference become gathers driven by
+// plan-time maps (swe_topology.cpp).  Fields live in halo-padded per-panel
+// arrays (HALO = 4 on each side, common pitch for all three meshes).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ch_swe.h"
+#include "swe_sbp.hpp"
+#include "swe_topology.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(SWE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+constexpr int HALO = 4;
+constexpr double PI = 3.14159265358979323846;
+
+// ------------------------------------------------------------------------
+// layout
+// ------------------------------------------------------------------------
+struct Dims {
+    int N;     // cells per panel edge
+    int P;     // row pitch (elements)
+    long PS;   // panel stride (elements)
+    long F;    // field stride = 6 * PS
+};
+
+Dims make_dims(int N)
+{
+    Dims d;
+    d.N = N;
+    d.P = ((N + 1 + 2 * HALO) + 15) / 16 * 16;
+    d.PS = (long)d.P * (N + 1 + 2 * HALO);
+    d.F = 6 * d.PS;
+    return d;
+}
+
+__host__ __device__ inline long off(const Dims &d, int p, int i, int j)
+{
+    return p * d.PS + (long)(j - 1 + HALO) * d.P + (i - 1 + HALO);
+}
+
+// flat interior index -> (p, i, j) for a mesh with nx x ny points per panel
+__device__ inline void unflat(long n, int nx, int ny, int &p, int &i, int &j)
+{
+    long per = (long)nx * ny;
+    p = (int)(n / per);
+    long r = n - p * per;
+    j = (int)(r / nx) + 1;
+    i = (int)(r - (long)(j - 1) * nx) + 1;
+}
+
+// ------------------------------------------------------------------------
+// geometry (FP64 on the device): metric/metric_2d_ecs_mod.f90 L338-411,
+// mesh/mesh_factory_mod.f90 L36-37, coriolis_factory_mod.f90 L239-280,
+// quadrature_factory_mod.f90 L89-188
+// ------------------------------------------------------------------------
+__constant__ int c_EX[6][3] = {{-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 0, 1}, {1, 0, 0}, {1, 0, 0}};
+__constant__ int c_EY[6][3] = {{0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, -1}};
+__constant__ int c_NN[6][3] = {{0, 0, 1}, {-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
+
+struct GeomParams {
+    double hx, radius, omega, axis[3];
+    int coriolis_const, order;
+};
+
+__device__ inline double sbp_weight(int idx, int n, bool centre, int order)
+{
+    // D21_A_interfaces/centers, D42_A_interfaces/centers
+    // (sbp_operators_collection_mod.f90 L169-170, L196-197)
+    const double i21[2] = {0.5, 1.0}, c21[2] = {1.0, 1.0};
+    const double i42[5] = {7.0 / 18, 9.0 / 8, 1.0, 71.0 / 72, 1.0};
+    const double c42[4] = {13.0 / 12, 7.0 / 8, 25.0 / 24, 1.0};
+    const double *A = order == 21 ? (centre ? c21 : i21) : (centre ? c42 : i42);
+    int ne = order == 21 ? 2 : (centre ? 4 : 5);
+    double w = 1.0;
+    if (idx <= ne) w = A[idx - 1];
+    if (n - idx + 1 <= ne) w = A[n - idx];
+    return w;
+}
+
+// mesh m: 0 xy, 1 u (x centres), 2 v (y centres).  Writes FP64 J, Qi, f.
+template <typename T>
+__global__ void k_metric(Dims d, GeomParams g, int m, T *c0, T *c1, T *c2, double *w)
+{
+    const int N = d.N;
+    const int xt = m == 1, yt = m == 2;
+    const int nx = xt ? N : N + 1, ny = yt ? N : N + 1;
+    long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= 6L * nx * ny) return;
+    int p, i, j;
+    unflat(n, nx, ny, p, i, j);
+    double al = -0.25 * PI + (i - 1 + 0.5 * xt) * g.hx;
+    double be = -0.25 * PI + (j - 1 + 0.5 * yt) * g.hx;
+    double X = tan(al), Y = tan(be);
+    double s2 = 1.0 + X * X + Y * Y, s = sqrt(s2);
+    double J = (1.0 + X * X) * (1.0 + Y * Y) / (s2 * s);
+    double q11 = s2 / (1.0 + X * X), q12 = s2 * X * Y / ((1.0 + X * X) * (1.0 + Y * Y)),
+           q22 = s2 / (1.0 + Y * Y);
+    long o = off(d, p, i, j);
+    w[o] = J * g.hx * g.hx * sbp_weight(i, nx, xt, g.order) * sbp_weight(j, ny, yt, g.order);
+    if (m == 0) {
+        // r = ex X/s + ey Y/s - n 1/s (ecs_proto2realface with identity rotation)
+        double r[3];
+        for (int k = 0; k < 3; k++) r[k] = (c_EX[p][k] * X + c_EY[p][k] * Y - c_NN[p][k]) / s;
+        double f;
+        if (g.coriolis_const) {
+            f = g.omega;
+        } else {
+            double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            f = 2.0 * g.omega * (g.axis[0] * r[0] + g.axis[1] * r[1] + g.axis[2] * r[2]) / rn;
+        }
+        c0[o] = T(J * q12);              // J Qi12 (co2contra cross term)
+        c1[o] = T(f * J * J);            // f J^2 (Coriolis)
+        c2[o] = T(1.0 / (J * g.radius * g.hx));  // 1/(hx J a) (div)
+    } else {
+        c0[o] = T(m == 1 ? q11 : q22);   // Qi11 at u, Qi22 at v
+        c1[o] = T(1.0 / J);
+        c2[o] = T(J);
+    }
+}
+
+// covariant edge-sync weights (ecs_Ah_vec_sync_factory_mod.f90 L97-152):
+// own across-edge covariant vector . neighbour's along-edge contravariant
+// vector, the neighbour's prototype face folded onto this one.
+__device__ inline void ecs_basis(double al, double be, double a1[3], double a2[3], double b1[3], double b2[3])
+{
+    double X = tan(al), Y = tan(be);
+    double s2 = 1 + X * X + Y * Y, s3 = s2 * sqrt(s2);
+    double cx = 1 + X * X, cy = 1 + Y * Y;
+    a1[0] = cx * cy / s3; a1[1] = -X * Y * cx / s3; a1[2] = -X * cx / s3;
+    a2[0] = -X * Y * cy / s3; a2[1] = cx * cy / s3; a2[2] = -Y * cy / s3;
+    double q11 = s2 / cx, q12 = s2 * X * Y / (cx * cy), q22 = s2 / cy;
+    for (int k = 0; k < 3; k++) {
+        b1[k] = q11 * a1[k] + q12 * a2[k];
+        b2[k] = q12 * a1[k] + q22 * a2[k];
+    }
+}
+
+template <typename T>
+__global__ void k_sync_weights(int N, double hx, T *qb, T *qt, T *ql, T *qr)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (i > N + 1) return;
+    double c = -0.25 * PI + (i - 1) * hx, e = 0.25 * PI;
+    double a1[3], a2[3], b1[3], b2[3], A1[3], A2[3], B1[3], B2[3];
+    ecs_basis(c, -e, a1, a2, b1, b2);
+    ecs_basis(c, e, A1, A2, B1, B2);
+    qb[i - 1] = T(a2[0] * B1[0] - a2[1] * B1[2] + a2[2] * B1[1]);
+    ecs_basis(c, e, a1, a2, b1, b2);
+    ecs_basis(c, -e, A1, A2, B1, B2);
+    qt[i - 1] = T(a2[0] * B1[0] + a2[1] * B1[2] - a2[2] * B1[1]);
+    ecs_basis(-e, c, a1, a2, b1, b2);
+    ecs_basis(e, c, A1, A2, B1, B2);
+    ql[i - 1] = T(-a1[0] * B2[2] + a1[1] * B2[1] + a1[2] * B2[0]);
+    ecs_basis(e, c, a1, a2, b1, b2);
+    ecs_basis(-e, c, A1, A2, B1, B2);
+    qr[i - 1] = T(a1[0] * B2[2] + a1[1] * B2[1] - a1[2] * B2[0]);
+}
+
+// ------------------------------------------------------------------------
+// operator kernels
+// ------------------------------------------------------------------------
+constexpr int TPB = 256;
+
+inline unsigned blocks_for(long n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+template <typename T, int O>
+__global__ void k_xy_w(Dims d, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ JQ12,
+                       T *__restrict__ W1, T *__restrict__ W2)
+{
+    const int N = d.N;
+    long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= 6L * (N + 1) * (N + 1)) return;
+    int p, i, j;
+    unflat(n, N + 1, N + 1, p, i, j);
+    long o = off(d, p, i, j);
+    T c = JQ12[o];
+    W1[o] = c * swe::i_c2i_pt<O>(v + off(d, p, i, 0), d.P, N, j);
+    W2[o] = c * swe::i_c2i_pt<O>(u + off(d, p, 0, j), 1, N, i);
+}
+
+template <typename T, int O>
+__global__ void k_contra(Dims d, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ W1,
+                         const T *__restrict__ W2, const T *__restrict__ Qu, const T *__restrict__ iJu,
+                         const T *__restrict__ Ju, const T *__restrict__ Qv, const T *__restrict__ iJv,
+                         const T *__restrict__ Jv, T *__restrict__ ut, T *__restrict__ vt, T *__restrict__ Jut,
+                         T *__restrict__ Jvt)
+{
+    const int N = d.N;
+    const long nu = 6L * N * (N + 1);
+    long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= 2 * nu) return;
+    int p, i, j;
+    if (n < nu) {
+        unflat(n, N, N + 1, p, i, j);
+        long o = off(d, p, i, j);
+        T val = Qu[o] * u[o] + swe::i_i2c_pt<O>(W1 + off(d, p, 0, j), 1, N, i) * iJu[o];
+        ut[o] = val;
+        Jut[o] = Ju[o] * val;
+    } else {
+        unflat(n - nu, N + 1, N, p, i, j);
+        long o = off(d, p, i, j);
+        T val = Qv[o] * v[o] + swe::i_i2c_pt<O>(W2 + off(d, p, i, 0), d.P, N, j) * iJv[o];
+        vt[o] = val;
+        Jvt[o] = Jv[o] * val;
+    }
+}
+
+// halo gather: dst_field[dst] = sign * src_field[src]
+struct GatherEntry {
+    int dst, src;
+    int fields;  // dst field index (bits 0-3), src field index (bits 4-7)
+    int sign;
+};
+struct FieldSet {
+    void *f[4];
+};
+
+template <typename T>
+__global__ void k_gather(const GatherEntry *__restrict__ e, int n, FieldSet fs)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    GatherEntry g = e[k];
+    T *dst = (T *)fs.f[g.fields & 15];
+    const T *src = (const T *)fs.f[(g.fields >> 4) & 15];
+    dst[g.dst] = T(g.sign) * src[g.src];
+}
+
+template <typename T, int O>
+__global__ void k_xy_cd(Dims d, const T *__restrict__ ut, const T *__restrict__ vt, const T *__restrict__ Jut,
+                        const T *__restrict__ Jvt, const T *__restrict__ fJ2, const T *__restrict__ iJa,
+                        T *__restrict__ up, T *__restrict__ vp, T *__restrict__ div)
+{
+    const int N = d.N;
+    long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= 6L * (N + 1) * (N + 1)) return;
+    int p, i, j;
+    unflat(n, N + 1, N + 1, p, i, j);
+    long o = off(d, p, i, j);
+    T f = fJ2[o];
+    // calc_cori_at_p_tile (coriolis_Ch_mod.f90 L97-120): (up,vp) <- f J^2 (vp, -up)
+    up[o] = f * swe::i_c2i_pt<O>(vt + off(d, p, i, 0), d.P, N, j);
+    vp[o] = -f * swe::i_c2i_pt<O>(ut + off(d, p, 0, j), 1, N, i);
+    // calc_div_tile (div_sbp_sat_mod.f90 L54-80), hx = hy
+    T dx = swe::d_c2i_pt<O>(Jut + off(d, p, 0, j), 1, N, i);
+    T dy = swe::d_c2i_pt<O>(Jvt + off(d, p, i, 0), d.P, N, j);
+    div[o] = (dx + dy) * iJa[o];
+}
+
+// boundary points of each panel, each once (corners included)
+__device__ inline void boundary_point(int N, int t, int &i, int &j)
+{
+    if (t < N) { i = t + 1; j = 1; }
+    else if (t < 2 * N) { i = N + 1; j = t - N + 1; }
+    else if (t < 3 * N) { i = N + 1 - (t - 2 * N); j = N + 1; }
+    else { i = 1; j = N + 1 - (t - 3 * N); }
+}
+
+template <typename T>
+__global__ void k_edge_sync(Dims d, T *__restrict__ up, T *__restrict__ vp, T *__restrict__ div,
+                            const T *__restrict__ qb, const T *__restrict__ qt, const T *__restrict__ ql,
+                            const T *__restrict__ qr)
+{
+    const int N = d.N, e = N + 1;
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= 6 * 4 * N) return;
+    int p = n / (4 * N), i, j;
+    boundary_point(N, n - p * 4 * N, i, j);
+    // covariant sync works on (U, V) = (vp, up)  (ecs_Ah_vec_sync_mod.f90 L51-53)
+#define U(a, b) vp[off(d, p, a, b)]
+#define V(a, b) up[off(d, p, a, b)]
+#define D(a, b) div[off(d, p, a, b)]
+    const T third = T(1.0 / 3.0), half = T(0.5);
+    T nu, nv, nd;
+    bool L = i == 1, R = i == e, B = j == 1, Tp = j == e;
+    if (L && B) {
+        nu = (U(1, 1) + U(0, 1) + U(1, 0) + qb[0] * V(1, 0)) * third;
+        nv = (V(1, 1) + V(0, 1) + V(1, 0) + ql[0] * U(0, 1)) * third;
+        nd = (D(1, 1) + D(0, 1) + D(1, 0)) * third;
+    } else if (R && B) {
+        nu = (U(e, 1) + U(e + 1, 1) + U(e, 0) + qb[e - 1] * V(e, 0)) * third;
+        nv = (V(e, 1) + V(e + 1, 1) + V(e, 0) + qr[0] * U(e + 1, 1)) * third;
+        nd = (D(e, 1) + D(e + 1, 1) + D(e, 0)) * third;
+    } else if (L && Tp) {
+        nu = (U(1, e) + U(0, e) + U(1, e + 1) + qt[0] * V(1, e + 1)) * third;
+        nv = (V(1, e) + V(0, e) + V(1, e + 1) + ql[e - 1] * U(0, e)) * third;
+        nd = (D(1, e) + D(0, e) + D(1, e + 1)) * third;
+    } else if (R && Tp) {
+        nu = (U(e, e) + U(e + 1, e) + U(e, e + 1) + qt[e - 1] * V(e, e + 1)) * third;
+        nv = (V(e, e) + V(e + 1, e) + V(e, e + 1) + qr[e - 1] * U(e + 1, e)) * third;
+        nd = (D(e, e) + D(e + 1, e) + D(e, e + 1)) * third;
+    } else if (L) {
+        nu = half * (U(0, j) + U(1, j));
+        nv = half * (V(1, j) + V(0, j) + ql[j - 1] * U(0, j));
+        nd = half * (D(0, j) + D(1, j));
+    } else if (R) {
+        nu = half * (U(e, j) + U(e + 1, j));
+        nv = half * (V(e, j) + V(e + 1, j) + qr[j - 1] * U(e + 1, j));
+        nd = half * (D(e, j) + D(e + 1, j));
+    } else if (B) {
+        nv = half * (V(i, 0) + V(i, 1));
+        nu = half * (U(i, 0) + U(i, 1) + qb[i - 1] * V(i, 0));
+        nd = half * (D(i, 1) + D(i, 0));
+    } else {
+        nv = half * (V(i, e) + V(i, e + 1));
+        nu = half * (U(i, e) + U(i, e + 1) + qt[i - 1] * V(i, e + 1));
+        nd = half * (D(i, e) + D(i, e + 1));
+    }
+    U(i, j) = nu;
+    V(i, j) = nv;
+    D(i, j) = nd;
+#undef U
+#undef V
+#undef D
+}
+
+template <typename T, int O>
+__global__ void k_tend(Dims d, const T *__restrict__ h, const T *__restrict__ up, const T *__restrict__ vp,
+                       const T *__restrict__ div, const T *__restrict__ iJu, const T *__restrict__ iJv, T g_hxa,
+                       T H0, T *__restrict__ du, T *__restrict__ dv, T *__restrict__ dh)
+{
+    const int N = d.N;
+    const long nu = 6L * N * (N + 1), nh = 6L * (N + 1) * (N + 1);
+    long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int p, i, j;
+    if (n < nu) {
+        unflat(n, N, N + 1, p, i, j);
+        long o = off(d, p, i, j);
+        du[o] = -g_hxa * swe::d_i2c_pt<O>(h + off(d, p, 0, j), 1, N, i) +
+                swe::i_i2c_pt<O>(up + off(d, p, 0, j), 1, N, i) * iJu[o];
+    } else if (n < 2 * nu) {
+        unflat(n - nu, N + 1, N, p, i, j);
+        long o = off(d, p, i, j);
+        dv[o] = -g_hxa * swe::d_i2c_pt<O>(h + off(d, p, i, 0), d.P, N, j) +
+                swe::i_i2c_pt<O>(vp + off(d, p, i, 0), d.P, N, j) * iJv[o];
+    } else if (n < 2 * nu + nh) {
+        unflat(n - 2 * nu, N + 1, N + 1, p, i, j);
+        long o = off(d, p, i, j);
+        dh[o] = -H0 * div[o];
+    }
+}
+
+// out = s1*x + s2*y (+ out if update), over whole state buffers
+template <typename T>
+__global__ void k_axpy(long n, T *__restrict__ out, T s1, const T *__restrict__ x, T s2, const T *__restrict__ y,
+                       int update)
+{
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    T r = s1 * x[k];
+    if (y) r += s2 * y[k];
+    out[k] = update ? out[k] + r : r;
+}
+
+// FP64 reduction over interior points: per-block partials in a fixed order
+template <typename T>
+__global__ void k_reduce(Dims d, const T *h, const T *u, const T *v, const T *ut, const T *vt, const double *wxy,
+                         const double *wu, const double *wv, double *part)
+{
+    const int N = d.N;
+    const long nu = 6L * N * (N + 1), nh = 6L * (N + 1) * (N + 1);
+    __shared__ double sm[3][TPB];
+    double a = 0, b = 0, c = 0;
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nh + 2 * nu; n += (long)gridDim.x * blockDim.x) {
+        int p, i, j;
+        if (n < nh) {
+            unflat(n, N + 1, N + 1, p, i, j);
+            long o = off(d, p, i, j);
+            double hv = (double)h[o];
+            a += hv * wxy[o];
+            b += hv * hv * wxy[o];
+        } else if (n < nh + nu) {
+            unflat(n - nh, N, N + 1, p, i, j);
+            long o = off(d, p, i, j);
+            c += (double)u[o] * (double)ut[o] * wu[o];
+        } else {
+            unflat(n - nh - nu, N + 1, N, p, i, j);
+            long o = off(d, p, i, j);
+            c += (double)v[o] * (double)vt[o] * wv[o];
+        }
+    }
+    sm[0][threadIdx.x] = a;
+    sm[1][threadIdx.x] = b;
+    sm[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int q = 0; q < 3; q++) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; q++) part[3 * blockIdx.x + q] = sm[q][0];
+}
+
+__global__ void k_reduce_final(const double *part, int nb, double *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s[3] = {0, 0, 0};
+    for (int k = 0; k < nb; k++)
+        for (int q = 0; q < 3; q++) s[q] += part[3 * k + q];
+    for (int q = 0; q < 3; q++) out[q] = s[q];
+}
+
+// ------------------------------------------------------------------------
+// engine
+// ------------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct EngineBase {
+    virtual ~EngineBase() = default;
+    virtual int apply(void *out, void *in) = 0;
+    virtual int axpy(void *out, double s1, void *x, double s2, void *y, int update) = 0;
+    virtual int integrals(void *st, double *mass, double *energy) = 0;
+    virtual size_t elem() const = 0;
+};
+
+struct Maps {
+    GatherEntry *xy = nullptr, *ch = nullptr, *xyv = nullptr;
+    int n_xy = 0, n_ch = 0, n_xyv = 0;
+};
+
+}  // namespace
+
+struct swe_plan {
+    swe_config cfg;
+    Dims d;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<EngineBase> eng;
+    swe_state *rk[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // k1..k4, y
+};
+
+struct swe_state {
+    swe_plan *plan;
+    void *buf;  // u, v, h fields back to back (3 * F elements)
+};
+
+namespace {
+
+template <typename T>
+struct Engine : EngineBase {
+    swe_plan *pl;
+    Dims d;
+    int order;
+    std::vector<void *> allocs;
+    T *JQ12, *fJ2, *iJa, *Qu, *iJu, *Ju, *Qv, *iJv, *Jv;
+    T *qb, *qt, *ql, *qr;
+    double *wxy, *wu, *wv;
+    T *W1, *W2, *ut, *vt, *Jut, *Jvt, *up, *vp, *div;
+    Maps maps;
+    double *part = nullptr, *red = nullptr;
+    int nblk_red = 512;
+
+    size_t elem() const override { return sizeof(T); }
+
+    ~Engine() override
+    {
+        for (void *p : allocs) (void)hipFree(p);
+    }
+
+    int alloc(void **p, size_t bytes)
+    {
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) return fail(SWE_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(*p);
+        e = hipMemsetAsync(*p, 0, bytes, pl->stream);
+        if (e != hipSuccess) return fail(SWE_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+        return SWE_OK;
+    }
+
+    template <typename U>
+    int field(U **p)
+    {
+        return alloc((void **)p, d.F * sizeof(U));
+    }
+
+    int upload_map(int kind, int depth, const std::vector<int> &fsel_dst, GatherEntry **out, int *count)
+    {
+        auto m = swe::build_halo_map(d.N, kind, depth);
+        std::vector<GatherEntry> ge(m.size());
+        for (size_t k = 0; k < m.size(); k++) {
+            const auto &e = m[k];
+            ge[k].dst = (int)off(d, e.tp, e.ti, e.tj);
+            ge[k].src = (int)off(d, e.sp, e.si, e.sj);
+            ge[k].fields = fsel_dst[e.tc] | (fsel_dst[e.sc] << 4);
+            ge[k].sign = e.sign;
+        }
+        int rc = alloc((void **)out, ge.size() * sizeof(GatherEntry));
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(*out, ge.data(), ge.size() * sizeof(GatherEntry), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        *count = (int)ge.size();
+        return SWE_OK;
+    }
+
+    int init(swe_plan *p)
+    {
+        pl = p;
+        d = p->d;
+        order = p->cfg.order;
+        int rc = 0;
+        T **cf[] = {&JQ12, &fJ2, &iJa, &Qu, &iJu, &Ju, &Qv, &iJv, &Jv,
+                    &W1, &W2, &ut, &vt, &Jut, &Jvt, &up, &vp, &div};
+        for (T **f : cf)
+            if ((rc = field(f))) return rc;
+        if ((rc = field(&wxy)) || (rc = field(&wu)) || (rc = field(&wv))) return rc;
+        T **qs[] = {&qb, &qt, &ql, &qr};
+        for (T **q : qs)
+            if ((rc = alloc((void **)q, (d.N + 1) * sizeof(T)))) return rc;
+        if ((rc = alloc((void **)&part, 3 * nblk_red * sizeof(double)))) return rc;
+        if ((rc = alloc((void **)&red, 4 * sizeof(double)))) return rc;
+
+        GeomParams g;
+        g.hx = 0.5 * PI / d.N;
+        g.radius = p->cfg.radius;
+        g.omega = p->cfg.omega;
+        for (int k = 0; k < 3; k++) g.axis[k] = p->cfg.axis[k];
+        g.coriolis_const = p->cfg.coriolis_const;
+        g.order = order;
+        long nxy = 6L * (d.N + 1) * (d.N + 1), nuv = 6L * d.N * (d.N + 1);
+        k_metric<T><<<blocks_for(nxy), TPB, 0, pl->stream>>>(d, g, 0, JQ12, fJ2, iJa, wxy);
+        k_metric<T><<<blocks_for(nuv), TPB, 0, pl->stream>>>(d, g, 1, Qu, iJu, Ju, wu);
+        k_metric<T><<<blocks_for(nuv), TPB, 0, pl->stream>>>(d, g, 2, Qv, iJv, Jv, wv);
+        k_sync_weights<T><<<blocks_for(d.N + 1), TPB, 0, pl->stream>>>(d.N, g.hx, qb, qt, ql, qr);
+        HIP_TRY(hipGetLastError());
+
+        // maps: field slots 0 and 1 are the (u-type, v-type) components
+        if ((rc = upload_map(0, 1, {0, 0}, &maps.xy, &maps.n_xy))) return rc;
+        if ((rc = upload_map(1, order == 21 ? 2 : 3, {0, 1}, &maps.ch, &maps.n_ch))) return rc;
+        if ((rc = upload_map(2, 1, {0, 1}, &maps.xyv, &maps.n_xyv))) return rc;
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        return SWE_OK;
+    }
+
+    void gather(const GatherEntry *m, int n, void *f0, void *f1)
+    {
+        FieldSet fs{{f0, f1, nullptr, nullptr}};
+        k_gather<T><<<blocks_for(n), TPB, 0, pl->stream>>>(m, n, fs);
+    }
+
+    template <int O>
+    void apply_o(T *out, T *in)
+    {
+        const int N = d.N;
+        T *u = in, *v = in + d.F, *h = in + 2 * d.F;
+        T *du = out, *dv = out + d.F, *dh = out + 2 * d.F;
+        long nxy = 6L * (N + 1) * (N + 1), nuv = 6L * N * (N + 1);
+        hipStream_t s = pl->stream;
+        gather(maps.xy, maps.n_xy, h, h);
+        k_xy_w<T, O><<<blocks_for(nxy), TPB, 0, s>>>(d, u, v, JQ12, W1, W2);
+        k_contra<T, O><<<blocks_for(2 * nuv), TPB, 0, s>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut, vt, Jut,
+                                                           Jvt);
+        gather(maps.ch, maps.n_ch, Jut, Jvt);
+        k_xy_cd<T, O><<<blocks_for(nxy), TPB, 0, s>>>(d, ut, vt, Jut, Jvt, fJ2, iJa, up, vp, div);
+        gather(maps.xyv, maps.n_xyv, up, vp);
+        gather(maps.xy, maps.n_xy, div, div);
+        k_edge_sync<T><<<blocks_for(24L * N), TPB, 0, s>>>(d, up, vp, div, qb, qt, ql, qr);
+        const swe_config &c = pl->cfg;
+        T g_hxa = T(c.grav / (0.5 * PI / N * c.radius));
+        k_tend<T, O><<<blocks_for(2 * nuv + nxy), TPB, 0, s>>>(d, h, up, vp, div, iJu, iJv, g_hxa, T(c.h_mean), du,
+                                                               dv, dh);
+    }
+
+    int apply(void *out, void *in) override
+    {
+        if (order == 21) apply_o<21>((T *)out, (T *)in);
+        else apply_o<42>((T *)out, (T *)in);
+        HIP_TRY(hipGetLastError());
+        return SWE_OK;
+    }
+
+    int axpy(void *out, double s1, void *x, double s2, void *y, int update) override
+    {
+        long n = 3 * d.F;
+        k_axpy<T><<<blocks_for(n), TPB, 0, pl->stream>>>(n, (T *)out, T(s1), (const T *)x, T(s2), (const T *)y,
+                                                          update);
+        HIP_TRY(hipGetLastError());
+        return SWE_OK;
+    }
+
+    int integrals(void *st, double *mass, double *energy) override
+    {
+        const int N = d.N;
+        T *u = (T *)st, *v = u + d.F, *h = u + 2 * d.F;
+        long nxy = 6L * (N + 1) * (N + 1), nuv = 6L * N * (N + 1);
+        hipStream_t s = pl->stream;
+        if (order == 21) {
+            k_xy_w<T, 21><<<blocks_for(nxy), TPB, 0, s>>>(d, u, v, JQ12, W1, W2);
+            k_contra<T, 21><<<blocks_for(2 * nuv), TPB, 0, s>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut, vt,
+                                                                Jut, Jvt);
+        } else {
+            k_xy_w<T, 42><<<blocks_for(nxy), TPB, 0, s>>>(d, u, v, JQ12, W1, W2);
+            k_contra<T, 42><<<blocks_for(2 * nuv), TPB, 0, s>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut, vt,
+                                                                Jut, Jvt);
+        }
+        k_reduce<T><<<nblk_red, TPB, 0, s>>>(d, h, u, v, ut, vt, wxy, wu, wv, part);
+        k_reduce_final<<<1, 64, 0, s>>>(part, nblk_red, red);
+        HIP_TRY(hipGetLastError());
+        double r[3];
+        HIP_TRY(hipMemcpyAsync(r, red, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        const swe_config &c = pl->cfg;
+        if (mass) *mass = r[0];
+        if (energy) *energy = 0.5 * c.grav * r[1] + 0.5 * c.h_mean * r[2];
+        return SWE_OK;
+    }
+};
+
+int check_state(swe_plan *plan, swe_state *s, const char *what)
+{
+    if (!s) return fail(SWE_EINVAL, std::string(what) + " is NULL");
+    if (s->plan != plan) return fail(SWE_EINVAL, std::string(what) + " belongs to another plan");
+    return SWE_OK;
+}
+
+}  // namespace
+
+namespace {
+template <typename T>
+void pack_to_padded(const Dims &d, const double *src, int nx, int ny, T *dst)
+{
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+                dst[off(d, p, i, j)] = T(src[((size_t)p * ny + (j - 1)) * nx + (i - 1)]);
+}
+template <typename T>
+void padded_to_pack(const Dims &d, const T *src, int nx, int ny, double *dst)
+{
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+                dst[((size_t)p * ny + (j - 1)) * nx + (i - 1)] = double(src[off(d, p, i, j)]);
+}
+template <typename T>
+int upload_t(swe_state *st, const double *u, const double *v, const double *h)
+{
+    const Dims &d = st->plan->d;
+    int N = d.N;
+    std::vector<T> host(3 * d.F, T(0));
+    pack_to_padded(d, u, N, N + 1, host.data());
+    pack_to_padded(d, v, N + 1, N, host.data() + d.F);
+    pack_to_padded(d, h, N + 1, N + 1, host.data() + 2 * d.F);
+    HIP_TRY(hipMemcpyAsync(st->buf, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice,
+                           st->plan->stream));
+    HIP_TRY(hipStreamSynchronize(st->plan->stream));
+    return SWE_OK;
+}
+template <typename T>
+int download_t(swe_state *st, double *u, double *v, double *h)
+{
+    const Dims &d = st->plan->d;
+    int N = d.N;
+    std::vector<T> host(3 * d.F);
+    HIP_TRY(hipMemcpyAsync(host.data(), st->buf, host.size() * sizeof(T), hipMemcpyDeviceToHost,
+                           st->plan->stream));
+    HIP_TRY(hipStreamSynchronize(st->plan->stream));
+    if (u) padded_to_pack(d, host.data(), N, N + 1, u);
+    if (v) padded_to_pack(d, host.data() + d.F, N + 1, N, v);
+    if (h) padded_to_pack(d, host.data() + 2 * d.F, N + 1, N + 1, h);
+    return SWE_OK;
+}
+}  // namespace
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+extern "C" {
+
+const char *swe_last_error(void) { return g_err.c_str(); }
+const char *swe_version(void) { return "ch_swe_amd 0.1 (gfx950)"; }
+
+int swe_plan_create(const swe_config *cfg, swe_plan **out)
+{
+    if (!cfg || !out) return fail(SWE_EINVAL, "swe_plan_create: NULL argument");
+    *out = nullptr;
+    if (cfg->order != 21 && cfg->order != 42) return fail(SWE_EINVAL, "order must be 21 or 42");
+    if (cfg->order == 21 && cfg->N < 4) return fail(SWE_EINVAL, "sbp21 needs N >= 4");
+    if (cfg->order == 42 && cfg->N < 8) return fail(SWE_EINVAL, "sbp42 needs N >= 8");
+    if (cfg->N > 4096) return fail(SWE_EINVAL, "N above 4096 is not supported");
+    if (cfg->precision != SWE_FP64 && cfg->precision != SWE_FP32)
+        return fail(SWE_EINVAL, "precision must be SWE_FP64 or SWE_FP32");
+    if (!(cfg->radius > 0) || !std::isfinite(cfg->grav) || !std::isfinite(cfg->h_mean) ||
+        !std::isfinite(cfg->omega))
+        return fail(SWE_EINVAL, "non-finite or non-positive physical constant");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SWE_EHIP, "no HIP device available");
+    auto *p = new swe_plan();
+    p->cfg = *cfg;
+    p->d = make_dims(cfg->N);
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete p;
+        return fail(SWE_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    int rc;
+    if (cfg->precision == SWE_FP64) {
+        auto *en = new Engine<double>();
+        p->eng.reset(en);
+        rc = en->init(p);
+    } else {
+        auto *en = new Engine<float>();
+        p->eng.reset(en);
+        rc = en->init(p);
+    }
+    for (int k = 0; k < 5 && rc == SWE_OK; k++) rc = swe_state_create(p, &p->rk[k]);
+    if (rc != SWE_OK) {
+        std::string msg = g_err;
+        swe_plan_destroy(p);
+        return fail(rc, msg);
+    }
+    *out = p;
+    return SWE_OK;
+}
+
+int swe_plan_destroy(swe_plan *plan)
+{
+    if (!plan) return SWE_OK;
+    if (plan->stream) (void)hipStreamSynchronize(plan->stream);
+    for (auto *&s : plan->rk) {
+        swe_state_destroy(s);
+        s = nullptr;
+    }
+    plan->eng.reset();
+    if (plan->stream) (void)hipStreamDestroy(plan->stream);
+    delete plan;
+    return SWE_OK;
+}
+
+int swe_plan_npts(const swe_plan *plan, int *n_u, int *n_v, int *n_h)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    int N = plan->d.N;
+    if (n_u) *n_u = 6 * N * (N + 1);
+    if (n_v) *n_v = 6 * N * (N + 1);
+    if (n_h) *n_h = 6 * (N + 1) * (N + 1);
+    return SWE_OK;
+}
+
+void *swe_plan_stream(swe_plan *plan) { return plan ? (void *)plan->stream : nullptr; }
+
+int swe_synchronize(swe_plan *plan)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    HIP_TRY(hipStreamSynchronize(plan->stream));
+    return SWE_OK;
+}
+
+int swe_state_create(swe_plan *plan, swe_state **out)
+{
+    if (!plan || !out) return fail(SWE_EINVAL, "swe_state_create: NULL argument");
+    size_t bytes = 3 * plan->d.F * plan->eng->elem();
+    void *buf = nullptr;
+    hipError_t e = hipMalloc(&buf, bytes);
+    if (e != hipSuccess) return fail(SWE_ENOMEM, std::string("hipMalloc(state): ") + hipGetErrorString(e));
+    HIP_TRY(hipMemsetAsync(buf, 0, bytes, plan->stream));
+    *out = new swe_state{plan, buf};
+    return SWE_OK;
+}
+
+int swe_state_destroy(swe_state *st)
+{
+    if (!st) return SWE_OK;
+    if (st->plan && st->plan->stream) (void)hipStreamSynchronize(st->plan->stream);
+    if (st->buf) (void)hipFree(st->buf);
+    delete st;
+    return SWE_OK;
+}
+
+int swe_state_upload(swe_state *st, const double *u, const double *v, const double *h)
+{
+    if (!st || !u || !v || !h) return fail(SWE_EINVAL, "swe_state_upload: NULL argument");
+    return st->plan->cfg.precision == SWE_FP64 ? upload_t<double>(st, u, v, h) : upload_t<float>(st, u, v, h);
+}
+
+int swe_state_download(swe_state *st, double *u, double *v, double *h)
+{
+    if (!st) return fail(SWE_EINVAL, "swe_state_download: NULL state");
+    return st->plan->cfg.precision == SWE_FP64 ? download_t<double>(st, u, v, h) : download_t<float>(st, u, v, h);
+}
+
+int swe_apply(swe_plan *plan, swe_state *vout, swe_state *vin)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    int rc;
+    if ((rc = check_state(plan, vout, "vout")) || (rc = check_state(plan, vin, "vin"))) return rc;
+    if (vout == vin) return fail(SWE_EINVAL, "swe_apply: vout and vin must differ");
+    return plan->eng->apply(vout->buf, vin->buf);
+}
+
+static int axpy_common(swe_plan *plan, swe_state *out, double s1, swe_state *v1, double s2, swe_state *v2,
+                       int update)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    int rc;
+    if ((rc = check_state(plan, out, "out")) || (rc = check_state(plan, v1, "v1"))) return rc;
+    if (v2 && (rc = check_state(plan, v2, "v2"))) return rc;
+    return plan->eng->axpy(out->buf, s1, v1->buf, s2, v2 ? v2->buf : nullptr, update);
+}
+
+int swe_assign(swe_plan *plan, swe_state *out, double s1, swe_state *v1, double s2, swe_state *v2)
+{
+    return axpy_common(plan, out, s1, v1, s2, v2, 0);
+}
+
+int swe_update(swe_plan *plan, swe_state *out, double s1, swe_state *v1, double s2, swe_state *v2)
+{
+    return axpy_common(plan, out, s1, v1, s2, v2, 1);
+}
+
+int swe_rk4_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    int rc = check_state(plan, v0, "v0");
+    if (rc) return rc;
+    if (nsteps < 0) return fail(SWE_EINVAL, "nsteps < 0");
+    if (!std::isfinite(dt)) return fail(SWE_EINVAL, "dt is not finite");
+    swe_state *k1 = plan->rk[0], *k2 = plan->rk[1], *k3 = plan->rk[2], *k4 = plan->rk[3], *y = plan->rk[4];
+    for (int it = 0; it < nsteps; it++) {
+        // rk4_mod.f90 step_rk4 L28-40
+        if ((rc = swe_apply(plan, k1, v0))) return rc;
+        if ((rc = swe_assign(plan, y, 1.0, v0, 0.5 * dt, k1))) return rc;
+        if ((rc = swe_apply(plan, k2, y))) return rc;
+        if ((rc = swe_assign(plan, y, 1.0, v0, 0.5 * dt, k2))) return rc;
+        if ((rc = swe_apply(plan, k3, y))) return rc;
+        if ((rc = swe_assign(plan, y, 1.0, v0, dt, k3))) return rc;
+        if ((rc = swe_apply(plan, k4, y))) return rc;
+        if ((rc = swe_update(plan, v0, dt / 6.0, k1, dt / 3.0, k2))) return rc;
+        if ((rc = swe_update(plan, v0, dt / 3.0, k3, dt / 6.0, k4))) return rc;
+    }
+    return SWE_OK;
+}
+
+int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    int rc = check_state(plan, st, "state");
+    if (rc) return rc;
+    return plan->eng->integrals(st->buf, mass, energy);
+}
+
+int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *tj, int *tc, int *sp, int *si,
+                 int *sj, int *sc, int *sign)
+{
+    if (!count) return fail(SWE_EINVAL, "NULL count");
+    if (N < 1 || (order != 21 && order != 42) || kind < 0 || kind > 2)
+        return fail(SWE_EINVAL, "swe_halo_map: bad arguments");
+    int depth = kind == 1 ? (order == 21 ? 2 : 3) : 1;
+    auto m = swe::build_halo_map(N, kind, depth);
+    *count = (int)m.size();
+    if (tp) {
+        for (size_t k = 0; k < m.size(); k++) {
+            tp[k] = m[k].tp; ti[k] = m[k].ti; tj[k] = m[k].tj; tc[k] = m[k].tc;
+            sp[k] = m[k].sp; si[k] = m[k].si; sj[k] = m[k].sj; sc[k] = m[k].sc;
+            sign[k] = m[k].sign;
+        }
+    }
+    return SWE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+// 1-D staggered SBP stencils evaluated at one output point (device side).
+//
+// f points at index 0 of a line (stride s, halo indices <= 0 and > n allowed
+// where the caller has filled them); n = number of cell centres on the line;
+// k = 1-based output index.  Coefficients follow
+//   differential_operators/sbp_operators/sbp_diff_i2c_{21,42}_mod.f90,
+//   sbp_diff_c2i_{21,42}_mod.f90, sbp_interp_i2c_{21,42}_mod.f90,
+//   sbp_interp_c2i_{21,42}_mod.f90,
+// restated as closure tables mirrored at the right end (derivatives change
+// sign under the mirror, interpolations do not).  SAT terms (derivatives
+// only) read the neighbour's duplicate / normal flux from the halo.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace swe {
+
+struct SBP42 {
+    // sbp_interp_i2c_42_mod.f90 L18-19
+    static constexpr double c13 = 0.1264926207034815;
+    static constexpr double c14 = -0.029892648949074313;
+};
+
+#define SWE_HD __host__ __device__ __forceinline__
+
+template <typename T>
+SWE_HD T ld(const T *f, int s, int k) { return f[(long)k * s]; }
+
+// ---- interfaces -> centres derivative (+ SAT with duplicates f[0], f[n+2])
+template <int ORDER, typename T>
+SWE_HD T d_i2c_pt(const T *f, int s, int n, int k)
+{
+    if (ORDER == 21) {
+        T v = ld(f, s, k + 1) - ld(f, s, k);
+        // 0.5*R/h with R = (3/2, -1/2), h = (1, 1)
+        if (k == 1) v += T(0.75) * (ld(f, s, 1) - ld(f, s, 0));
+        if (k == 2) v += T(-0.25) * (ld(f, s, 1) - ld(f, s, 0));
+        if (k == n) v += T(0.75) * (ld(f, s, n + 2) - ld(f, s, n + 1));
+        if (k == n - 1) v += T(-0.25) * (ld(f, s, n + 2) - ld(f, s, n + 1));
+        return v;
+    } else {
+        // closure rows (q11..q35) and SAT weights 0.5*R_r/h_r
+        constexpr double Q[3][5] = {{-79.0 / 78, 27.0 / 26, -1.0 / 26, 1.0 / 78, 0},
+                                    {2.0 / 21, -9.0 / 7, 9.0 / 7, -2.0 / 21, 0},
+                                    {1.0 / 75, 0, -27.0 / 25, 83.0 / 75, -1.0 / 25}};
+        constexpr double S[3] = {0.5 * (15.0 / 8) / (13.0 / 12), 0.5 * (-5.0 / 4) / (7.0 / 8),
+                                 0.5 * (3.0 / 8) / (25.0 / 24)};
+        if (k <= 3) {
+            const int r = k - 1;
+            T v = T(Q[r][0]) * ld(f, s, 1) + T(Q[r][1]) * ld(f, s, 2) + T(Q[r][2]) * ld(f, s, 3) +
+                  T(Q[r][3]) * ld(f, s, 4) + T(Q[r][4]) * ld(f, s, 5);
+            v += T(S[r]) * (ld(f, s, 1) - ld(f, s, 0));
+            return v;
+        }
+        if (k >= n - 2) {
+            const int r = n - k;
+            T v = -(T(Q[r][0]) * ld(f, s, n + 1) + T(Q[r][1]) * ld(f, s, n) + T(Q[r][2]) * ld(f, s, n - 1) +
+                    T(Q[r][3]) * ld(f, s, n - 2) + T(Q[r][4]) * ld(f, s, n - 3));
+            v += T(S[r]) * (ld(f, s, n + 2) - ld(f, s, n + 1));
+            return v;
+        }
+        return (T(27) * (ld(f, s, k + 1) - ld(f, s, k)) - (ld(f, s, k + 2) - ld(f, s, k - 1))) * T(1.0 / 24);
+    }
+}
+
+// ---- centres -> interfaces derivative (+ SAT with neighbour's normal flux
+//      at depths 1..ORDER/21+1 stored in f[0], f[-1], ... and f[n+1], ...)
+template <int ORDER, typename T>
+SWE_HD T d_c2i_pt(const T *f, int s, int n, int k)
+{
+    const int nx = n + 1;
+    if (ORDER == 21) {
+        T v;
+        if (k == 1) v = ld(f, s, 2) - ld(f, s, 1);
+        else if (k == nx) v = ld(f, s, nx - 1) - ld(f, s, nx - 2);
+        else v = ld(f, s, k) - ld(f, s, k - 1);
+        // 0.5/h11 = 1; R = (3/2, -1/2)
+        if (k == 1)
+            v += T(1.5) * (ld(f, s, 1) - ld(f, s, 0)) - T(0.5) * (ld(f, s, 2) - ld(f, s, -1));
+        if (k == nx)
+            v -= T(1.5) * (ld(f, s, nx - 1) - ld(f, s, nx)) - T(0.5) * (ld(f, s, nx - 2) - ld(f, s, nx + 1));
+        return v;
+    } else {
+        constexpr double Q[4][5] = {{-2.0, 3.0, -1.0, 0, 0},
+                                    {-1.0, 1.0, 0, 0, 0},
+                                    {1.0 / 24, -9.0 / 8, 9.0 / 8, -1.0 / 24, 0},
+                                    {-1.0 / 71, 6.0 / 71, -83.0 / 71, 81.0 / 71, -3.0 / 71}};
+        constexpr double C = 0.5 / (7.0 / 18);
+        constexpr double R[3] = {15.0 / 8, -5.0 / 4, 3.0 / 8};
+        if (k <= 4) {
+            const int r = k - 1;
+            T v = T(Q[r][0]) * ld(f, s, 1) + T(Q[r][1]) * ld(f, s, 2) + T(Q[r][2]) * ld(f, s, 3) +
+                  T(Q[r][3]) * ld(f, s, 4) + T(Q[r][4]) * ld(f, s, 5);
+            if (k == 1)
+                v += T(C) * (T(R[0]) * (ld(f, s, 1) - ld(f, s, 0)) + T(R[1]) * (ld(f, s, 2) - ld(f, s, -1)) +
+                             T(R[2]) * (ld(f, s, 3) - ld(f, s, -2)));
+            return v;
+        }
+        if (k >= nx - 3) {
+            const int r = nx - k;
+            T v = -(T(Q[r][0]) * ld(f, s, nx - 1) + T(Q[r][1]) * ld(f, s, nx - 2) + T(Q[r][2]) * ld(f, s, nx - 3) +
+                    T(Q[r][3]) * ld(f, s, nx - 4) + T(Q[r][4]) * ld(f, s, nx - 5));
+            if (k == nx)
+                v -= T(C) * (T(R[0]) * (ld(f, s, nx - 1) - ld(f, s, nx)) +
+                             T(R[1]) * (ld(f, s, nx - 2) - ld(f, s, nx + 1)) +
+                             T(R[2]) * (ld(f, s, nx - 3) - ld(f, s, nx + 2)));
+            return v;
+        }
+        return (T(27) * (ld(f, s, k) - ld(f, s, k - 1)) - (ld(f, s, k + 1) - ld(f, s, k - 2))) * T(1.0 / 24);
+    }
+}
+
+// ---- interfaces -> centres interpolation (no halo reads)
+template <int ORDER, typename T>
+SWE_HD T i_i2c_pt(const T *f, int s, int n, int k)
+{
+    if (ORDER == 21) return T(0.5) * (ld(f, s, k + 1) + ld(f, s, k));
+    constexpr double c13 = SBP42::c13, c14 = SBP42::c14;
+    constexpr double Q[3][5] = {
+        {0.5 + c13 + 2 * c14, 0.5 - 2 * c13 - 3 * c14, c13, c14, 0},
+        {-8.0 / 63 - 52.0 / 21 * c13 - 104.0 / 21 * c14, 29.0 / 42 + 104.0 / 21 * c13 + 52.0 / 7 * c14,
+         -52.0 / 21 * c13 + 0.5, -52.0 / 21 * c14 - 4.0 / 63, 0},
+        {26.0 / 25 * c13 + 52.0 / 25 * c14 - 1.0 / 25, -1.0 / 50 - 52.0 / 25 * c13 - 78.0 / 25 * c14,
+         3.0 / 5 + 26.0 / 25 * c13, 13.0 / 25 + 26.0 / 25 * c14, -3.0 / 50}};
+    if (k <= 3) {
+        const int r = k - 1;
+        return T(Q[r][0]) * ld(f, s, 1) + T(Q[r][1]) * ld(f, s, 2) + T(Q[r][2]) * ld(f, s, 3) +
+               T(Q[r][3]) * ld(f, s, 4) + T(Q[r][4]) * ld(f, s, 5);
+    }
+    if (k >= n - 2) {
+        const int r = n - k;
+        return T(Q[r][0]) * ld(f, s, n + 1) + T(Q[r][1]) * ld(f, s, n) + T(Q[r][2]) * ld(f, s, n - 1) +
+               T(Q[r][3]) * ld(f, s, n - 2) + T(Q[r][4]) * ld(f, s, n - 3);
+    }
+    return (T(9) * (ld(f, s, k + 1) + ld(f, s, k)) - (ld(f, s, k + 2) + ld(f, s, k - 1))) * T(1.0 / 16);
+}
+
+// ---- centres -> interfaces interpolation (no halo reads)
+template <int ORDER, typename T>
+SWE_HD T i_c2i_pt(const T *f, int s, int n, int k)
+{
+    const int nx = n + 1;
+    if (ORDER == 21) {
+        if (k == 1) return ld(f, s, 1);
+        if (k == nx) return ld(f, s, nx - 1);
+        return T(0.5) * (ld(f, s, k) + ld(f, s, k - 1));
+    }
+    constexpr double c13 = SBP42::c13, c14 = SBP42::c14;
+    constexpr double Q[4][5] = {
+        {39.0 / 28 + 39.0 / 14 * c13 + 39.0 / 7 * c14, -2.0 / 7 - 39.0 / 7 * c13 - 78.0 / 7 * c14,
+         39.0 / 14 * c13 + 39.0 / 7 * c14 - 3.0 / 28, 0, 0},
+        {13.0 / 27 - 52.0 / 27 * c13 - 26.0 / 9 * c14, 29.0 / 54 + 104.0 / 27 * c13 + 52.0 / 9 * c14,
+         -1.0 / 54 - 52.0 / 27 * c13 - 26.0 / 9 * c14, 0, 0},
+        {13.0 / 12 * c13, -13.0 / 6 * c13 + 7.0 / 16, 5.0 / 8 + 13.0 / 12 * c13, -1.0 / 16, 0},
+        {78.0 / 71 * c14, -156.0 / 71 * c14 - 4.0 / 71, 39.0 / 71 + 78.0 / 71 * c14, 81.0 / 142, -9.0 / 142}};
+    if (k <= 4) {
+        const int r = k - 1;
+        return T(Q[r][0]) * ld(f, s, 1) + T(Q[r][1]) * ld(f, s, 2) + T(Q[r][2]) * ld(f, s, 3) +
+               T(Q[r][3]) * ld(f, s, 4) + T(Q[r][4]) * ld(f, s, 5);
+    }
+    if (k >= nx - 3) {
+        const int r = nx - k;
+        return T(Q[r][0]) * ld(f, s, nx - 1) + T(Q[r][1]) * ld(f, s, nx - 2) + T(Q[r][2]) * ld(f, s, nx - 3) +
+               T(Q[r][3]) * ld(f, s, nx - 4) + T(Q[r][4]) * ld(f, s, nx - 5);
+    }
+    return (T(9) * (ld(f, s, k) + ld(f, s, k - 1)) - (ld(f, s, k + 1) + ld(f, s, k - 2))) * T(1.0 / 16);
+}
+
+}  // namespace swe

@@ -1,0 +1,111 @@
+/*
+ * ch_swe_amd -- MI355X-native linear shallow-water RK4 step on the staggered
+ * (Ch) equiangular cubed sphere with SBP operators of order 21 and 42.
+ *
+ * C ABI of libchswe.so.  Plain pointers and sizes only.  Every entry point
+ * returns 0 on success or a negative SWE_E* code; swe_last_error() gives the
+ * message of the last failure on the calling thread.  A plan owns one HIP
+ * stream (default priority); calls on one plan are serialised by the caller
+ * (one plan per host thread), as the reference runs one thread per MPI rank.
+ *
+ * Host field layout ("packed"): FP64, one array per field, panels 1..6, each
+ * panel stored as the reference's tile array p(i,j) without halos
+ * (column-major: i fastest):
+ *   u : mesh 'y'  [6][N+1][N]      covariant u at (i=1..N,   j=1..N+1)
+ *   v : mesh 'x'  [6][N][N+1]      covariant v at (i=1..N+1, j=1..N)
+ *   h : mesh 'xy' [6][N+1][N+1]    height perturbation at vertices
+ * (reference: domain/domain_factory_mod.f90 L297-313, partition_mod.f90 L83-97)
+ */
+#ifndef CH_SWE_H
+#define CH_SWE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SWE_OK = 0,
+    SWE_EINVAL = -1,   /* bad argument (size, order, null pointer, mismatch) */
+    SWE_EHIP = -2,     /* HIP runtime error (no device, launch failure, ...) */
+    SWE_ENOMEM = -3,   /* device allocation failed */
+    SWE_ENOTFINITE = -4 /* non-finite value detected in a checked result */
+};
+
+enum { SWE_FP64 = 0, SWE_FP32 = 1 };
+
+typedef struct swe_plan swe_plan;
+typedef struct swe_state swe_state;
+
+/* Operator configuration.  Mirrors the reference's domain/metric config
+ * (N, planet_radius, omega, rotation_axis, coriolis_constant:
+ * metric/metric_factory_mod.f90 L98-202) and create_linear_swm_operator's
+ * arguments (grav, h_mean, SBP order via the operator names:
+ * models/shallow_water/operator/operator_swm_factory_mod.f90 L586-685). */
+typedef struct {
+    int N;              /* cells per panel edge; >= 4 (sbp21) or >= 8 (sbp42) */
+    int order;          /* 21 or 42 */
+    int precision;      /* SWE_FP64 or SWE_FP32 (stencils/state); metric is FP64 */
+    int coriolis_const; /* 1: f = omega everywhere; 0: f = 2 omega (axis . r) */
+    double radius;      /* planet radius [m] */
+    double grav;        /* gravity [m/s^2] */
+    double h_mean;      /* mean depth H0 [m] */
+    double omega;       /* rotation rate [1/s] */
+    double axis[3];     /* rotation axis */
+} swe_config;
+
+const char *swe_last_error(void);
+const char *swe_version(void);
+
+/* Plan: geometry (FP64 metric built on the device), edge maps, work fields,
+ * stream.  Replaces create_domain + create_linear_swm_operator +
+ * create_timescheme("rk4") (models/shallow_water/shallow_water_mod.f90
+ * L70-93). */
+int swe_plan_create(const swe_config *cfg, swe_plan **out);
+int swe_plan_destroy(swe_plan *plan);
+int swe_plan_npts(const swe_plan *plan, int *n_u, int *n_v, int *n_h);
+void *swe_plan_stream(swe_plan *plan); /* the plan's hipStream_t */
+int swe_synchronize(swe_plan *plan);
+
+/* State vector (u, v, h) resident on the device: stvec_swm_t
+ * (models/shallow_water/stvec/stvec_swm_mod.f90 L10-42) and
+ * stvec_t%create_similar (stvec/stvec_mod.f90 L10-30). */
+int swe_state_create(swe_plan *plan, swe_state **out);
+int swe_state_destroy(swe_state *st);
+int swe_state_upload(swe_state *st, const double *u, const double *v, const double *h);
+int swe_state_download(swe_state *st, double *u, double *v, double *h);
+
+/* operator_t%apply (src/operator_mod.f90 L11-41) for operator_swm_lin_t
+ * (operator_swm_lin_mod.f90 apply L66-108): vout = A(vin). */
+int swe_apply(swe_plan *plan, swe_state *vout, swe_state *vin);
+
+/* stvec_t%assign / %update (stvec/stvec_mod.f90 L10-30; interior points,
+ * grid_field_mod.f90 L460-534):
+ *   assign: out = s1*v1 + s2*v2      (v2 may be NULL: out = s1*v1)
+ *   update: out = out + s1*v1 + s2*v2 (v2 may be NULL) */
+int swe_assign(swe_plan *plan, swe_state *out, double s1, swe_state *v1, double s2, swe_state *v2);
+int swe_update(swe_plan *plan, swe_state *out, double s1, swe_state *v1, double s2, swe_state *v2);
+
+/* rk4_t%step (time_schemes/rk4_mod.f90 L20-42) applied nsteps times to v0 in
+ * place.  The plan owns the k1..k4, y registers (timescheme_factory_mod.f90
+ * L90-107). */
+int swe_rk4_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps);
+
+/* FP64 SBP-quadrature integrals on the device (quadrature/sbp_quadrature_mod.f90
+ * L22-62, abstract_quadrature_mod.f90 L29-66): mass = sum h J hx hy Ax Ay;
+ * energy = g/2 <h,h> + H0/2 (<u,ut> + <v,vt>). */
+int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy);
+
+/* Host-only (no device needed): halo map of the in-device edge gathers, for
+ * inspection and tests.  kind 0: xy scalar depth 1; 1: Ch vector (u across
+ * left/right, v across bottom/top) depth 2 (sbp21) or 3 (sbp42); 2: xy
+ * vector depth 1.  Same entry convention as the exchange builders of
+ * parallel/exchange_factory_mod.f90 L166-672: target (panel 0..5, i, j,
+ * component) <- sign * source (panel, i, j, component).  Pass NULL arrays to
+ * query the count. */
+int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *tj, int *tc,
+                 int *sp, int *si, int *sj, int *sc, int *sign);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,181 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  FP64 bar: 1e-12 relative (max error / max magnitude of
+the field).  FP32 bar: stated per test.  Full-size cases (N=384) are checked
+through size-independent invariants instead of the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import ch_swe_amd as cs
+from oracle.swe_oracle import MESH_U, MESH_V, MESH_XY
+
+pytestmark = pytest.mark.gpu
+
+CENTRES = [(np.pi, 0.0),                                           # panel interior
+           (np.pi / 4, np.arcsin(1 / np.sqrt(3))),                 # cube corner
+           (0.27 * np.pi, np.arcsin(1 / np.sqrt(3)) - 0.02 * np.pi)]  # 1 cell off
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _random_state(o, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(o.shape(MESH_U)), rng.standard_normal(o.shape(MESH_V)),
+            rng.standard_normal(o.shape(MESH_XY)))
+
+
+def _op(N, order, precision=cs.FP64, **kw):
+    return cs.create_linear_swm_operator(N, order, precision, **kw)
+
+
+@pytest.mark.parametrize("order,N", [(21, 8), (21, 16), (21, 33), (42, 8), (42, 16), (42, 40)])
+@pytest.mark.parametrize("cor_const", [False, True])
+def test_apply_fp64_matches_oracle(oracle_mod, order, N, cor_const):
+    _say(f"apply fp64 order={order} N={N} cor_const={cor_const}")
+    kw = dict(coriolis_const=cor_const, omega=1e-4 if cor_const else 7.292e-5)
+    o = oracle_mod.Oracle(N, order, **kw)
+    u, v, h = _random_state(o, 1000 + N)
+    ref = o.apply(u, v, h)
+    op = _op(N, order, **kw)
+    vin, vout = op.create_stvec(), op.create_stvec()
+    vin.upload(u, v, h)
+    op.apply(vout, vin)
+    got = vout.download()
+    for g, r in zip(got, ref):
+        assert _rel(g, r) < 1e-12
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_rk4_config1_fp64_matches_oracle(oracle_mod, seed):
+    """N=32, sbp21, FP64, a=6.371e6, H0=1000, g=9.81, f=2 Omega sin(phi),
+    h = 10 m exp(-(d/500 km)^2) at the seed's centre, u=v=0, dt=600 s, 100
+    RK4 steps."""
+    lam0, phi0 = CENTRES[seed % 3]
+    _say(f"rk4 config-1 seed={seed} centre=({lam0:.4f},{phi0:.4f})")
+    o = oracle_mod.Oracle(32, 21)
+    h0 = o.gaussian_hill(lam0, phi0, 5e5 / o.radius, 10.0)
+    u0, v0 = o.zeros(MESH_U), o.zeros(MESH_V)
+    ref = o.rk4(u0, v0, h0, 600.0, 100)
+    op = _op(32, 21)
+    st = op.create_stvec().upload(u0, v0, h0)
+    cs.create_timescheme("rk4").step(st, op, 600.0, 100)
+    got = st.download()
+    for g, r in zip(got, ref):
+        assert _rel(g, r) < 1e-12
+
+
+@pytest.mark.parametrize("order", [21, 42])
+def test_rk4_fp64_random_start_matches_oracle(oracle_mod, order):
+    N = 24
+    _say(f"rk4 random start order={order}")
+    o = oracle_mod.Oracle(N, order)
+    u, v, h = _random_state(o, 7)
+    u, v = 10 * u, 10 * v
+    ref = o.rk4(u, v, h, 300.0, 20)
+    op = _op(N, order)
+    st = op.create_stvec().upload(u, v, h)
+    cs.create_timescheme("rk4").step(st, op, 300.0, 20)
+    for g, r in zip(st.download(), ref):
+        assert _rel(g, r) < 1e-12
+
+
+@pytest.mark.parametrize("order", [21, 42])
+def test_apply_fp32_close_to_oracle(oracle_mod, order):
+    """FP32 stencils with FP64-built coefficients: one apply on smooth input
+    agrees with FP64 to 1e-5 relative (max error over max magnitude; dh
+    involves cancellation in the divergence)."""
+    N = 32
+    _say(f"apply fp32 order={order}")
+    o = oracle_mod.Oracle(N, order)
+    h = sum(o.gaussian_hill(lam, phi, 0.4, 10.0) for lam, phi in CENTRES)
+    ut = o.covariant_from_cart(*(0 * o.zeros(MESH_U) + c for c in (1.0, 2.0, 0.5)), MESH_U)
+    vt = o.covariant_from_cart(*(0 * o.zeros(MESH_V) + c for c in (1.0, 2.0, 0.5)), MESH_V)
+    ref = o.apply(ut, vt, h)
+    op = _op(N, order, cs.FP32)
+    vin, vout = op.create_stvec(), op.create_stvec()
+    vin.upload(ut, vt, h)
+    op.apply(vout, vin)
+    for g, r in zip(vout.download(), ref):
+        assert _rel(g, r) < 1e-5
+
+
+def test_stvec_assign_update(oracle_mod):
+    """stvec_t%assign / %update semantics (grid_field_mod.f90 L460-534)."""
+    N = 12
+    o = oracle_mod.Oracle(N, 21)
+    a = _random_state(o, 11)
+    b = _random_state(o, 12)
+    op = _op(N, 21)
+    A, B, C = op.create_stvec(), op.create_stvec(), op.create_stvec()
+    A.upload(*a)
+    B.upload(*b)
+    C.assign(2.0, A, -3.0, B)
+    for g, x, y in zip(C.download(), a, b):
+        assert np.abs(g - (2 * x - 3 * y)).max() < 1e-14
+    C.update(0.5, A)
+    for g, x, y in zip(C.download(), a, b):
+        assert np.abs(g - (2 * x - 3 * y + 0.5 * x)).max() < 1e-14
+    C.assign(4.0, B)
+    for g, y in zip(C.download(), b):
+        assert np.abs(g - 4 * y).max() < 1e-15
+
+
+@pytest.mark.parametrize("order", [21, 42])
+def test_integrals_match_oracle(oracle_mod, order):
+    N = 20
+    o = oracle_mod.Oracle(N, order)
+    u, v, h = _random_state(o, 21)
+    op = _op(N, order)
+    st = op.create_stvec().upload(u, v, h)
+    m, e = op.integrals(st)
+    assert abs(m - o.mass(h)) < 1e-12 * np.abs(h).max() * 4 * np.pi
+    assert abs(e / o.energy(u, v, h) - 1) < 1e-12
+
+
+def test_errors_are_reported():
+    op = _op(8, 21)
+    other = _op(8, 21)
+    a = op.create_stvec()
+    b = other.create_stvec()
+    with pytest.raises(cs.SweError, match="another plan"):
+        op.apply(a, b)
+    with pytest.raises(cs.SweError, match="must differ"):
+        op.apply(a, a)
+
+
+@pytest.mark.parametrize("precision", [cs.FP64, cs.FP32])
+def test_full_size_invariants(oracle_mod, precision):
+    """N=384, sbp42: the discrete operator conserves mass exactly and the
+    SBP energy to roundoff (for continuous h); checked on the device output
+    with the oracle's FP64 quadrature weights only (no oracle apply)."""
+    N = 384
+    _say(f"full-size invariants N={N} precision={precision}")
+    o = oracle_mod.Oracle(N, 42)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(o.shape(MESH_U))
+    v = rng.standard_normal(o.shape(MESH_V))
+    h = sum(o.gaussian_hill(lam, phi, 0.3, 10.0) for lam, phi in CENTRES)
+    op = _op(N, 42, precision)
+    vin, vout = op.create_stvec(), op.create_stvec()
+    vin.upload(u, v, h)
+    op.apply(vout, vin)
+    du, dv, dh = vout.download()
+    wu, wv, wh = o.weights(MESH_U), o.weights(MESH_V), o.weights(MESH_XY)
+    tol = 1e-12 if precision == cs.FP64 else 2e-5
+    assert abs(np.sum(dh * wh)) < tol * np.sqrt(np.sum(dh * dh * wh) * np.sum(wh))
+    # energy rate with the device's own co2contra: E = g/2<h,h> + H0/2<u,Cu>;
+    # dE = g<h,dh> + H0<Cu, du>.  Cu from the oracle (FP64, same inputs).
+    ut, vt = o.co2contra(u, v)
+    dE = o.grav * np.sum(h * dh * wh) + o.h_mean * (np.sum(ut * du * wu) + np.sum(vt * dv * wv))
+    scale = (o.grav * np.sqrt(np.sum(h * h * wh) * np.sum(dh * dh * wh))
+             + o.h_mean * np.sqrt((np.sum(ut * ut * wu) + np.sum(vt * vt * wv))
+                                  * (np.sum(du * du * wu) + np.sum(dv * dv * wv))))
+    assert abs(dE / scale) < (1e-12 if precision == cs.FP64 else 1e-5)
