@@ -50,7 +50,8 @@ class _CConfig(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("order", ctypes.c_int), ("precision", ctypes.c_int),
                 ("coriolis_const", ctypes.c_int), ("radius", ctypes.c_double),
                 ("grav", ctypes.c_double), ("h_mean", ctypes.c_double),
-                ("omega", ctypes.c_double), ("axis", ctypes.c_double * 3)]
+                ("omega", ctypes.c_double), ("axis", ctypes.c_double * 3),
+                ("flags", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -107,8 +108,9 @@ class Config:
 
     def __init__(self, N, order=42, precision=FP64, radius=6.371e6, grav=9.81,
                  h_mean=1000.0, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
-                 coriolis_const=False):
+                 coriolis_const=False, unfused=False):
         self.N, self.order, self.precision = int(N), int(order), int(precision)
+        self.unfused = bool(unfused)
         self.radius, self.grav, self.h_mean, self.omega = radius, grav, h_mean, omega
         self.axis = tuple(float(a) for a in axis)
         self.coriolis_const = bool(coriolis_const)
@@ -119,6 +121,7 @@ class Config:
         c.coriolis_const = 1 if self.coriolis_const else 0
         c.radius, c.grav, c.h_mean, c.omega = self.radius, self.grav, self.h_mean, self.omega
         c.axis[:] = self.axis
+        c.flags = 1 if self.unfused else 0
         return c
 
 
@@ -218,11 +221,12 @@ class Rk4:
 
 def create_linear_swm_operator(N, order=42, precision=FP64, grav=9.81, h_mean=1000.0,
                                radius=6.371e6, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
-                               coriolis_const=False):
+                               coriolis_const=False, unfused=False):
     """Factory mirroring create_linear_swm_operator
-    (operator_swm_factory_mod.f90 L586-685) for the Ch staggering."""
+    (operator_swm_factory_mod.f90 L586-685) for the Ch staggering.
+    unfused=True selects the multi-kernel diagnostic path."""
     return OperatorSwmLin(Config(N, order, precision, radius, grav, h_mean, omega, axis,
-                                 coriolis_const))
+                                 coriolis_const, unfused))
 
 
 def create_timescheme(name):

@@ -18,15 +18,44 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    cmd = [HIPCC] + FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", LIB + ".tmp"]
+FORTRAN = os.path.join(HERE, "fortran")
+FC = os.environ.get("FC_FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_SOURCES = ["swe_hip_capi_mod.f90", "swe_hip_mod.f90"]
+FORTRAN_PROGRAMS = {"swe_hip_gaussian": "swe_hip_gaussian_main.f90"}
+
+
+def _run(cmd, what, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libchswe.so")
+        raise RuntimeError(f"{what} failed")
+
+
+def build_fortran(verbose=False):
+    """The Fortran host layer and driver programs (flang, linked to
+    libchswe.so with an $ORIGIN rpath).  Skipped when flang is absent."""
+    if not os.path.exists(FC):
+        return None
+    mod_dir = os.path.join(FORTRAN, "_build")
+    os.makedirs(mod_dir, exist_ok=True)
+    out = []
+    for prog, main in FORTRAN_PROGRAMS.items():
+        exe = os.path.join(HERE, prog)
+        srcs = [os.path.join(FORTRAN, f) for f in FORTRAN_SOURCES + [main]]
+        _run([FC, "-O2", "-module-dir", mod_dir] + srcs +
+             ["-L" + HERE, "-lchswe", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"], f"flang {prog}", verbose)
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
+def build(verbose=False):
+    cmd = [HIPCC] + FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", LIB + ".tmp"]
+    _run(cmd, "hipcc (libchswe.so)", verbose)
     os.replace(LIB + ".tmp", LIB)
+    build_fortran(verbose)
     return LIB
 
 

@@ -23,6 +23,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -213,8 +214,8 @@ __global__ void k_xy_w(Dims d, const T *__restrict__ u, const T *__restrict__ v,
     unflat(n, N + 1, N + 1, p, i, j);
     long o = off(d, p, i, j);
     T c = JQ12[o];
-    W1[o] = c * swe::i_c2i_pt<O>(v + off(d, p, i, 0), d.P, N, j);
-    W2[o] = c * swe::i_c2i_pt<O>(u + off(d, p, 0, j), 1, N, i);
+    W1[o] = c * swe::i_c2i_pt<O, T>(swe::line(v + off(d, p, i, 0), d.P), 0, N, j);
+    W2[o] = c * swe::i_c2i_pt<O, T>(swe::line(u + off(d, p, 0, j), 1), 0, N, i);
 }
 
 template <typename T, int O>
@@ -232,13 +233,13 @@ __global__ void k_contra(Dims d, const T *__restrict__ u, const T *__restrict__ 
     if (n < nu) {
         unflat(n, N, N + 1, p, i, j);
         long o = off(d, p, i, j);
-        T val = Qu[o] * u[o] + swe::i_i2c_pt<O>(W1 + off(d, p, 0, j), 1, N, i) * iJu[o];
+        T val = Qu[o] * u[o] + swe::i_i2c_pt<O, T>(swe::line(W1 + off(d, p, 0, j), 1), 0, N, i) * iJu[o];
         ut[o] = val;
         Jut[o] = Ju[o] * val;
     } else {
         unflat(n - nu, N + 1, N, p, i, j);
         long o = off(d, p, i, j);
-        T val = Qv[o] * v[o] + swe::i_i2c_pt<O>(W2 + off(d, p, i, 0), d.P, N, j) * iJv[o];
+        T val = Qv[o] * v[o] + swe::i_i2c_pt<O, T>(swe::line(W2 + off(d, p, i, 0), d.P), 0, N, j) * iJv[o];
         vt[o] = val;
         Jvt[o] = Jv[o] * val;
     }
@@ -278,11 +279,11 @@ __global__ void k_xy_cd(Dims d, const T *__restrict__ ut, const T *__restrict__ 
     long o = off(d, p, i, j);
     T f = fJ2[o];
     // calc_cori_at_p_tile (coriolis_Ch_mod.f90 L97-120): (up,vp) <- f J^2 (vp, -up)
-    up[o] = f * swe::i_c2i_pt<O>(vt + off(d, p, i, 0), d.P, N, j);
-    vp[o] = -f * swe::i_c2i_pt<O>(ut + off(d, p, 0, j), 1, N, i);
+    up[o] = f * swe::i_c2i_pt<O, T>(swe::line(vt + off(d, p, i, 0), d.P), 0, N, j);
+    vp[o] = -f * swe::i_c2i_pt<O, T>(swe::line(ut + off(d, p, 0, j), 1), 0, N, i);
     // calc_div_tile (div_sbp_sat_mod.f90 L54-80), hx = hy
-    T dx = swe::d_c2i_pt<O>(Jut + off(d, p, 0, j), 1, N, i);
-    T dy = swe::d_c2i_pt<O>(Jvt + off(d, p, i, 0), d.P, N, j);
+    T dx = swe::d_c2i_pt<O, T>(swe::line(Jut + off(d, p, 0, j), 1), 0, N, i);
+    T dy = swe::d_c2i_pt<O, T>(swe::line(Jvt + off(d, p, i, 0), d.P), 0, N, j);
     div[o] = (dx + dy) * iJa[o];
 }
 
@@ -365,13 +366,13 @@ __global__ void k_tend(Dims d, const T *__restrict__ h, const T *__restrict__ up
     if (n < nu) {
         unflat(n, N, N + 1, p, i, j);
         long o = off(d, p, i, j);
-        du[o] = -g_hxa * swe::d_i2c_pt<O>(h + off(d, p, 0, j), 1, N, i) +
-                swe::i_i2c_pt<O>(up + off(d, p, 0, j), 1, N, i) * iJu[o];
+        du[o] = -g_hxa * swe::d_i2c_pt<O, T>(swe::line(h + off(d, p, 0, j), 1), 0, N, i) +
+                swe::i_i2c_pt<O, T>(swe::line(up + off(d, p, 0, j), 1), 0, N, i) * iJu[o];
     } else if (n < 2 * nu) {
         unflat(n - nu, N + 1, N, p, i, j);
         long o = off(d, p, i, j);
-        dv[o] = -g_hxa * swe::d_i2c_pt<O>(h + off(d, p, i, 0), d.P, N, j) +
-                swe::i_i2c_pt<O>(vp + off(d, p, i, 0), d.P, N, j) * iJv[o];
+        dv[o] = -g_hxa * swe::d_i2c_pt<O, T>(swe::line(h + off(d, p, i, 0), d.P), 0, N, j) +
+                swe::i_i2c_pt<O, T>(swe::line(vp + off(d, p, i, 0), d.P), 0, N, j) * iJv[o];
     } else if (n < 2 * nu + nh) {
         unflat(n - 2 * nu, N + 1, N + 1, p, i, j);
         long o = off(d, p, i, j);
@@ -440,6 +441,60 @@ __global__ void k_reduce_final(const double *part, int nb, double *out)
     for (int q = 0; q < 3; q++) out[q] = s[q];
 }
 
+#include "swe_fused.inc"
+
+// panel-independent coefficient tables for the fused path (FP64 arithmetic,
+// stored in T) on the (N+1) x (N+1) index grid, plus f J^2 per panel
+template <typename T>
+__global__ void k_coef(int N, GeomParams g, T *JQ12, T *iJa, T *Q11, T *iJu, T *Ju, T *Q22, T *iJv, T *Jv, T *fJ2)
+{
+    const int n = N + 1;
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= 6L * n * n) return;
+    int p = (int)(k / ((long)n * n));
+    int r = (int)(k - (long)p * n * n);
+    int j = r / n + 1, i = r % n + 1;
+    auto metric = [&](double xs, double ys, double &J, double &q11, double &q12, double &q22, double &X,
+                      double &Y, double &s) {
+        double al = -0.25 * PI + (i - 1 + xs) * g.hx, be = -0.25 * PI + (j - 1 + ys) * g.hx;
+        X = tan(al);
+        Y = tan(be);
+        double s2 = 1.0 + X * X + Y * Y;
+        s = sqrt(s2);
+        J = (1.0 + X * X) * (1.0 + Y * Y) / (s2 * s);
+        q11 = s2 / (1.0 + X * X);
+        q12 = s2 * X * Y / ((1.0 + X * X) * (1.0 + Y * Y));
+        q22 = s2 / (1.0 + Y * Y);
+    };
+    double J, q11, q12, q22, X, Y, s;
+    metric(0.0, 0.0, J, q11, q12, q22, X, Y, s);
+    double f;
+    if (g.coriolis_const) {
+        f = g.omega;
+    } else {
+        double rr[3];
+        for (int m = 0; m < 3; m++) rr[m] = (c_EX[p][m] * X + c_EY[p][m] * Y - c_NN[p][m]) / s;
+        double rn = sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
+        f = 2.0 * g.omega * (g.axis[0] * rr[0] + g.axis[1] * rr[1] + g.axis[2] * rr[2]) / rn;
+    }
+    fJ2[k] = T(f * J * J);
+    if (p != 0) return;
+    JQ12[r] = T(J * q12);
+    iJa[r] = T(1.0 / (J * g.radius * g.hx));
+    if (i <= N) {
+        metric(0.5, 0.0, J, q11, q12, q22, X, Y, s);
+        Q11[r] = T(q11);
+        iJu[r] = T(1.0 / J);
+        Ju[r] = T(J);
+    }
+    if (j <= N) {
+        metric(0.0, 0.5, J, q11, q12, q22, X, Y, s);
+        Q22[r] = T(q22);
+        iJv[r] = T(1.0 / J);
+        Jv[r] = T(J);
+    }
+}
+
 // ------------------------------------------------------------------------
 // engine
 // ------------------------------------------------------------------------
@@ -456,6 +511,8 @@ struct EngineBase {
     virtual int apply(void *out, void *in) = 0;
     virtual int axpy(void *out, double s1, void *x, double s2, void *y, int update) = 0;
     virtual int integrals(void *st, double *mass, double *energy) = 0;
+    virtual int rk4(void *v0, void *acc, void *ya, void *yb, double dt, int nsteps) = 0;
+    virtual bool fused() const = 0;
     virtual size_t elem() const = 0;
 };
 
@@ -495,7 +552,17 @@ struct Engine : EngineBase {
     double *part = nullptr, *red = nullptr;
     int nblk_red = 512;
 
+    // fused path
+    static constexpr int TY = sizeof(T) == 4 ? 16 : 8;
+    bool use_fused = true;
+    Coef<T> cf{};
+    T *rec[2] = {nullptr, nullptr};
+    int ntx = 0, nty = 0, ntiles = 0, nseg = 0;
+    int *tile_segs = nullptr, *seg_expect = nullptr, *seg_cnt = nullptr;
+    int4 *seg_desc = nullptr;
+
     size_t elem() const override { return sizeof(T); }
+    bool fused() const override { return use_fused; }
 
     ~Engine() override
     {
@@ -572,7 +639,173 @@ struct Engine : EngineBase {
         if ((rc = upload_map(0, 1, {0, 0}, &maps.xy, &maps.n_xy))) return rc;
         if ((rc = upload_map(1, order == 21 ? 2 : 3, {0, 1}, &maps.ch, &maps.n_ch))) return rc;
         if ((rc = upload_map(2, 1, {0, 1}, &maps.xyv, &maps.n_xyv))) return rc;
+        use_fused = !(p->cfg.flags & 1);
+        if ((rc = init_fused(g))) return rc;
         HIP_TRY(hipStreamSynchronize(pl->stream));
+        return SWE_OK;
+    }
+
+    int init_fused(const GeomParams &g)
+    {
+        const int N = d.N, n = N + 1;
+        int rc;
+        // coefficient tables
+        T *tabs[8];
+        for (auto &t : tabs)
+            if ((rc = alloc((void **)&t, (size_t)n * n * sizeof(T)))) return rc;
+        T *tf;
+        if ((rc = alloc((void **)&tf, 6 * (size_t)n * n * sizeof(T)))) return rc;
+        k_coef<T><<<blocks_for(6L * n * n), TPB, 0, pl->stream>>>(N, g, tabs[0], tabs[1], tabs[2], tabs[3], tabs[4],
+                                                                  tabs[5], tabs[6], tabs[7], tf);
+        HIP_TRY(hipGetLastError());
+        cf = Coef<T>{tabs[0], tabs[1], tabs[2], tabs[3], tabs[4], tabs[5], tabs[6], tabs[7], tf, qb, qt, ql, qr};
+        for (auto &r : rec)
+            if ((rc = alloc((void **)&r, (size_t)24 * REC_KINDS * n * sizeof(T)))) return rc;
+        // cube side table (constant memory, identical for every plan)
+        SideDesc sd[6][4];
+        for (int q = 0; q < 6; q++)
+            for (int sdi = 0; sdi < 4; sdi++) {
+                swe::SideMatch m = swe::match_side(q, sdi);
+                sd[q][sdi] = SideDesc{m.q, m.q_side, m.reversed ? 1 : 0};
+            }
+        HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_side), sd, sizeof(sd)));
+        // tiles, edge segments and which tiles write into each segment's band
+        ntx = (n + F_TX - 1) / F_TX;
+        nty = (n + TY - 1) / TY;
+        ntiles = 6 * ntx * nty;
+        auto ext = [n](int lo, int hi, int &a, int &b) {  // ext_rg for region 0 (m = 14)
+            for (int pass = 0; pass < 2; pass++) {
+                lo = lo <= 3 ? 1 : lo;
+                hi = hi >= n - 2 ? n : hi;
+                if (hi >= n) lo = std::min(lo, n - 13);
+                if (lo <= 1) hi = std::max(hi, 14);
+            }
+            a = std::max(lo, 1);
+            b = std::min(hi, n);
+        };
+        struct Box { int x0, x1, y0, y1; };
+        std::vector<Box> tiles(ntiles);
+        for (int p = 0; p < 6; p++)
+            for (int ty = 0; ty < nty; ty++)
+                for (int tx = 0; tx < ntx; tx++) {
+                    int i0 = 1 + tx * F_TX, j0 = 1 + ty * TY;
+                    tiles[(p * nty + ty) * ntx + tx] = Box{i0, std::min(n, i0 + F_TX - 1), j0, std::min(n, j0 + TY - 1)};
+                }
+        std::vector<int4> segs;
+        std::vector<Box> bands;
+        std::vector<int> segp;
+        for (int p = 0; p < 6; p++)
+            for (int side = 0; side < 4; side++) {
+                int cnt = side <= 1 ? nty : ntx;
+                for (int k = 0; k < cnt; k++) {
+                    int lo = side <= 1 ? 1 + k * TY : 1 + k * F_TX;
+                    int hi = std::min(n, side <= 1 ? lo + TY - 1 : lo + F_TX - 1);
+                    segs.push_back(make_int4(p, side, lo, hi));
+                    Box b;
+                    int bx0, bx1, by0, by1;
+                    if (side == 0) { bx0 = 1; bx1 = 4; by0 = lo; by1 = hi; }
+                    else if (side == 1) { bx0 = n - 3; bx1 = n; by0 = lo; by1 = hi; }
+                    else if (side == 2) { bx0 = lo; bx1 = hi; by0 = 1; by1 = 4; }
+                    else { bx0 = lo; bx1 = hi; by0 = n - 3; by1 = n; }
+                    ext(bx0 - 6, bx1 + 6, b.x0, b.x1);
+                    ext(by0 - 6, by1 + 6, b.y0, b.y1);
+                    bands.push_back(b);
+                    segp.push_back(p);
+                }
+            }
+        nseg = (int)segs.size();
+        std::vector<int> ts((size_t)ntiles * F_MAXSEG, -1), expect(nseg, 0);
+        for (int t = 0; t < ntiles; t++) {
+            int p = t / (ntx * nty), cnt = 0;
+            const Box &a = tiles[t];
+            for (int k = 0; k < nseg; k++) {
+                if (segp[k] != p) continue;
+                const Box &b = bands[k];
+                if (a.x1 < b.x0 || b.x1 < a.x0 || a.y1 < b.y0 || b.y1 < a.y0) continue;
+                if (cnt == F_MAXSEG) return fail(SWE_EINVAL, "internal: tile touches too many edge segments");
+                ts[(size_t)t * F_MAXSEG + cnt++] = k;
+                expect[k]++;
+            }
+        }
+        if ((rc = alloc((void **)&tile_segs, ts.size() * sizeof(int)))) return rc;
+        if ((rc = alloc((void **)&seg_expect, expect.size() * sizeof(int)))) return rc;
+        if ((rc = alloc((void **)&seg_cnt, expect.size() * sizeof(int)))) return rc;
+        if ((rc = alloc((void **)&seg_desc, segs.size() * sizeof(int4)))) return rc;
+        HIP_TRY(hipMemcpyAsync(tile_segs, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(seg_expect, expect.data(), expect.size() * sizeof(int), hipMemcpyHostToDevice,
+                               pl->stream));
+        HIP_TRY(hipMemcpyAsync(seg_desc, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        return SWE_OK;
+    }
+
+    template <int O>
+    StageArgs<T> stage_args(const T *y) const
+    {
+        StageArgs<T> a{};
+        a.d = d;
+        a.c = cf;
+        a.y = y;
+        const swe_config &c = pl->cfg;
+        a.g_hxa = T(c.grav / (0.5 * PI / d.N * c.radius));
+        a.H0 = T(c.h_mean);
+        a.ntx = ntx;
+        a.nty = nty;
+        a.tile_segs = tile_segs;
+        a.seg_expect = seg_expect;
+        a.seg_desc = seg_desc;
+        a.seg_cnt = seg_cnt;
+        return a;
+    }
+
+    template <int O>
+    void launch_records(const T *y, T *r)
+    {
+        k_records<T, O, TY><<<nseg, F_THREADS, 0, pl->stream>>>(d, cf, y, r, seg_desc);
+    }
+
+    template <int O>
+    void fused_apply(T *out, const T *in)
+    {
+        launch_records<O>(in, rec[0]);
+        StageArgs<T> a = stage_args<O>(in);
+        a.out = out;
+        a.mode = 0;
+        a.rec_in = rec[0];
+        a.produce = 0;
+        k_stage<T, O, TY><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
+    }
+
+    template <int O>
+    void fused_rk4(T *v0, T *acc, T *ya, T *yb, double dt, int nsteps)
+    {
+        launch_records<O>(v0, rec[0]);
+        const T *ins[4] = {v0, ya, yb, ya};
+        T *nexts[4] = {ya, yb, ya, nullptr};
+        const double A[4] = {0.5 * dt, 0.5 * dt, dt, 0.0};
+        const double B[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
+        for (int it = 0; it < nsteps; it++)
+            for (int st = 0; st < 4; st++) {
+                StageArgs<T> a = stage_args<O>(ins[st]);
+                a.v0 = v0;
+                a.acc = acc;
+                a.ynext = nexts[st];
+                a.out = v0;
+                a.a_dt = T(A[st]);
+                a.b_dt = T(B[st]);
+                a.mode = st == 0 ? 1 : (st == 3 ? 3 : 2);
+                a.rec_in = rec[st & 1];
+                a.rec_out = rec[(st + 1) & 1];
+                a.produce = 1;
+                k_stage<T, O, TY><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
+            }
+    }
+
+    int rk4(void *v0, void *acc, void *ya, void *yb, double dt, int nsteps) override
+    {
+        if (order == 21) fused_rk4<21>((T *)v0, (T *)acc, (T *)ya, (T *)yb, dt, nsteps);
+        else fused_rk4<42>((T *)v0, (T *)acc, (T *)ya, (T *)yb, dt, nsteps);
+        HIP_TRY(hipGetLastError());
         return SWE_OK;
     }
 
@@ -607,7 +840,10 @@ struct Engine : EngineBase {
 
     int apply(void *out, void *in) override
     {
-        if (order == 21) apply_o<21>((T *)out, (T *)in);
+        if (use_fused) {
+            if (order == 21) fused_apply<21>((T *)out, (const T *)in);
+            else fused_apply<42>((T *)out, (const T *)in);
+        } else if (order == 21) apply_o<21>((T *)out, (T *)in);
         else apply_o<42>((T *)out, (T *)in);
         HIP_TRY(hipGetLastError());
         return SWE_OK;
@@ -859,7 +1095,9 @@ int swe_rk4_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps)
     if (rc) return rc;
     if (nsteps < 0) return fail(SWE_EINVAL, "nsteps < 0");
     if (!std::isfinite(dt)) return fail(SWE_EINVAL, "dt is not finite");
+    if (nsteps == 0) return SWE_OK;
     swe_state *k1 = plan->rk[0], *k2 = plan->rk[1], *k3 = plan->rk[2], *k4 = plan->rk[3], *y = plan->rk[4];
+    if (plan->eng->fused()) return plan->eng->rk4(v0->buf, k1->buf, k2->buf, k3->buf, dt, nsteps);
     for (int it = 0; it < nsteps; it++) {
         // rk4_mod.f90 step_rk4 L28-40
         if ((rc = swe_apply(plan, k1, v0))) return rc;

@@ -1,8 +1,9 @@
 // 1-D staggered SBP stencils evaluated at one output point (device side).
 //
-// f points at index 0 of a line (stride s, halo indices <= 0 and > n allowed
-// where the caller has filled them); n = number of cell centres on the line;
-// k = 1-based output index.  Coefficients follow
+// f is a line accessor: f(k) returns the value at 1-based line index k
+// (halo indices <= 0 and > n are requested only by SAT terms, where the
+// caller provides the neighbour's values); n = number of cell centres on the
+// line; k = 1-based output index.  Coefficients follow
 //   differential_operators/sbp_operators/sbp_diff_i2c_{21,42}_mod.f90,
 //   sbp_diff_c2i_{21,42}_mod.f90, sbp_interp_i2c_{21,42}_mod.f90,
 //   sbp_interp_c2i_{21,42}_mod.f90,
@@ -23,12 +24,22 @@ struct SBP42 {
 
 #define SWE_HD __host__ __device__ __forceinline__
 
+// strided pointer line: element k at p[k * s] (p points at index 0)
 template <typename T>
-SWE_HD T ld(const T *f, int s, int k) { return f[(long)k * s]; }
+struct Line {
+    const T *p;
+    long s;
+    SWE_HD T operator()(int k) const { return p[k * s]; }
+};
+template <typename T>
+SWE_HD Line<T> line(const T *p, long s) { return Line<T>{p, s}; }
+
+template <typename A>
+SWE_HD auto ld(const A &f, int, int k) -> decltype(f(k)) { return f(k); }
 
 // ---- interfaces -> centres derivative (+ SAT with duplicates f[0], f[n+2])
-template <int ORDER, typename T>
-SWE_HD T d_i2c_pt(const T *f, int s, int n, int k)
+template <int ORDER, typename T, typename A>
+SWE_HD T d_i2c_pt(const A &f, int s, int n, int k)
 {
     if (ORDER == 21) {
         T v = ld(f, s, k + 1) - ld(f, s, k);
@@ -65,8 +76,8 @@ SWE_HD T d_i2c_pt(const T *f, int s, int n, int k)
 
 // ---- centres -> interfaces derivative (+ SAT with neighbour's normal flux
 //      at depths 1..ORDER/21+1 stored in f[0], f[-1], ... and f[n+1], ...)
-template <int ORDER, typename T>
-SWE_HD T d_c2i_pt(const T *f, int s, int n, int k)
+template <int ORDER, typename T, typename A>
+SWE_HD T d_c2i_pt(const A &f, int s, int n, int k)
 {
     const int nx = n + 1;
     if (ORDER == 21) {
@@ -111,8 +122,8 @@ SWE_HD T d_c2i_pt(const T *f, int s, int n, int k)
 }
 
 // ---- interfaces -> centres interpolation (no halo reads)
-template <int ORDER, typename T>
-SWE_HD T i_i2c_pt(const T *f, int s, int n, int k)
+template <int ORDER, typename T, typename A>
+SWE_HD T i_i2c_pt(const A &f, int s, int n, int k)
 {
     if (ORDER == 21) return T(0.5) * (ld(f, s, k + 1) + ld(f, s, k));
     constexpr double c13 = SBP42::c13, c14 = SBP42::c14;
@@ -136,8 +147,8 @@ SWE_HD T i_i2c_pt(const T *f, int s, int n, int k)
 }
 
 // ---- centres -> interfaces interpolation (no halo reads)
-template <int ORDER, typename T>
-SWE_HD T i_c2i_pt(const T *f, int s, int n, int k)
+template <int ORDER, typename T, typename A>
+SWE_HD T i_c2i_pt(const A &f, int s, int n, int k)
 {
     const int nx = n + 1;
     if (ORDER == 21) {

@@ -51,6 +51,7 @@ typedef struct {
     double h_mean;      /* mean depth H0 [m] */
     double omega;       /* rotation rate [1/s] */
     double axis[3];     /* rotation axis */
+    int flags;          /* bit 0: run the unfused multi-kernel path (diagnostics) */
 } swe_config;
 
 const char *swe_last_error(void);

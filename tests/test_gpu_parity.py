@@ -36,7 +36,14 @@ def _op(N, order, precision=cs.FP64, **kw):
     return cs.create_linear_swm_operator(N, order, precision, **kw)
 
 
-@pytest.mark.parametrize("order,N", [(21, 8), (21, 16), (21, 33), (42, 8), (42, 16), (42, 40)])
+# sizes chosen to exercise the fused tiling: single-tile panels (N=8), last
+# tile one point wide in y (FP64 tiles are 32x8, FP32 32x16: N=16, 32) and
+# in x (N=32), exact multiples (N=15, 31, 47), and larger panels
+APPLY_CASES = [(21, 8), (21, 16), (21, 32), (21, 33), (42, 8), (42, 15), (42, 16),
+               (42, 31), (42, 40), (42, 47)]
+
+
+@pytest.mark.parametrize("order,N", APPLY_CASES)
 @pytest.mark.parametrize("cor_const", [False, True])
 def test_apply_fp64_matches_oracle(oracle_mod, order, N, cor_const):
     _say(f"apply fp64 order={order} N={N} cor_const={cor_const}")
@@ -50,6 +57,24 @@ def test_apply_fp64_matches_oracle(oracle_mod, order, N, cor_const):
     op.apply(vout, vin)
     got = vout.download()
     for g, r in zip(got, ref):
+        assert _rel(g, r) < 1e-12
+
+
+@pytest.mark.parametrize("order,N", [(21, 16), (42, 16), (42, 33)])
+def test_unfused_path_matches_oracle(oracle_mod, order, N):
+    """The multi-kernel diagnostic path (halo gathers + point kernels)."""
+    _say(f"apply unfused order={order} N={N}")
+    o = oracle_mod.Oracle(N, order)
+    u, v, h = _random_state(o, 77)
+    op = _op(N, order, unfused=True)
+    vin, vout = op.create_stvec(), op.create_stvec()
+    vin.upload(u, v, h)
+    op.apply(vout, vin)
+    for g, r in zip(vout.download(), o.apply(u, v, h)):
+        assert _rel(g, r) < 1e-12
+    st = op.create_stvec().upload(u, v, h)
+    cs.create_timescheme("rk4").step(st, op, 300.0, 3)
+    for g, r in zip(st.download(), o.rk4(u, v, h, 300.0, 3)):
         assert _rel(g, r) < 1e-12
 
 
@@ -72,10 +97,9 @@ def test_rk4_config1_fp64_matches_oracle(oracle_mod, seed):
         assert _rel(g, r) < 1e-12
 
 
-@pytest.mark.parametrize("order", [21, 42])
-def test_rk4_fp64_random_start_matches_oracle(oracle_mod, order):
-    N = 24
-    _say(f"rk4 random start order={order}")
+@pytest.mark.parametrize("order,N", [(21, 24), (42, 24), (42, 40)])
+def test_rk4_fp64_random_start_matches_oracle(oracle_mod, order, N):
+    _say(f"rk4 random start order={order} N={N}")
     o = oracle_mod.Oracle(N, order)
     u, v, h = _random_state(o, 7)
     u, v = 10 * u, 10 * v
