@@ -11,8 +11,8 @@ steps with the state resident in HBM, timed with HIP events on the plan's
 stream (and wall clock around it, both bracketed by device synchronisation).
 The same steps are run in FP64; the line reports both rates, the FP32-vs-FP64
 relative L2 error of (h, u, v) after warmup+steps, and the energy drift.
-cpu_baseline: the CPU oracle (scalar C, FP64, one core) on a bounded sample of
-the same workload.
+cpu_baseline: the CPU oracle (C, FP64, OpenMP row loops on the host cores) on
+a bounded sample of the same workload.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -88,24 +88,39 @@ def rel_l2(a, b, w):
     return float(np.sqrt(np.sum(w * (a - b) ** 2) / np.sum(w * b * b)))
 
 
-def cpu_baseline(N, state0, dt, nsteps):
-    from oracle.swe_oracle import Oracle
+def cpu_baseline(N, state0, dt, nsteps, threads):
+    """The CPU oracle (C, FP64, OpenMP over rows) on nsteps of the same run."""
+    from oracle import swe_oracle
 
-    o = Oracle(N, ORDER)
+    swe_oracle.set_num_threads(threads)
+    o = swe_oracle.Oracle(N, ORDER)
     t = time.perf_counter()
     o.rk4(*state0, dt, nsteps)
     sec = time.perf_counter() - t
-    return 6 * N * N * nsteps / sec / 1e6, sec
+    return 6 * N * N * nsteps / sec / 1e6, sec, swe_oracle.num_threads()
+
+
+def host_threads():
+    """Threads for the CPU baseline: OMP_NUM_THREADS if set, else the cores
+    this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--N", type=int, default=N_DEFAULT)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-threads", type=int, default=host_threads())
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (3 launches) so that a "
+                         "counter profile can scale FETCH_SIZE / WRITE_SIZE by a known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -123,6 +138,19 @@ def main():
 
     import ch_swe_amd as cs
 
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy calibration: 3 x {args.copy_calibration} MiB read + write", file=sys.stderr,
+              flush=True)
+        del src, dst
+
     ev = HipEvents()
     r32 = run_gpu(cs, cs.FP32, N, state0, dt, args.steps, args.warmup, ev)
     print(f"# fp32: {r32['ms']:.3f} ms for {args.steps} steps", file=sys.stderr, flush=True)
@@ -134,15 +162,24 @@ def main():
     cells = 6 * N * N
     ms_step = r32["ms"] / args.steps
     value = cells * args.steps / (r32["ms"] * 1e-3) / 1e6
-    step_bytes = 16 * dof(N) * 4
-    achieved = step_bytes / (ms_step * 1e-3) / 1e9
+    # Dominant kernel: the fused stage kernel, 4 launches per step.  Its
+    # algorithmic traffic is 3, 5, 5, 3 state vectors for the four stages
+    # (stage 1 reads y writes acc, y2; stages 2-3 read y, v0, acc and write
+    # acc, y'; stage 4 reads y, acc and writes v0): 4 vectors per launch on
+    # average, one vector = DOF * 4 bytes.  The timed region holds the 4*K
+    # stage launches plus one edge-record launch for the initial state, whose
+    # duration (a few us) is charged to the stage launches.
+    launch_bytes = 4 * dof(N) * 4
+    launch_us = r32["ms"] * 1e3 / (4 * args.steps)
+    achieved = launch_bytes / (launch_us * 1e-6) / 1e9
 
     errs = None
     if r64 is not None:
         w = {m: tc.sbp_weights(N, m, ORDER) for m in ("u", "v", "h")}
         errs = {m: rel_l2(a, b, w[m]) for m, a, b in zip(("u", "v", "h"), r32["state"], r64["state"])}
 
-    cpu_val, cpu_sec = cpu_baseline(N, state0, dt, args.cpu_steps)
+    cpu_val, cpu_sec, cpu_threads = (cpu_baseline(N, state0, dt, args.cpu_steps, args.cpu_threads)
+                                     if args.cpu_steps > 0 else (None, 0.0, 0))
 
     line = {
         "metric": "Mpts*steps/s (FP32 stencils, FP64 metric and integrals), N=384 sbp42 single solve",
@@ -170,7 +207,9 @@ def main():
             "energy_drift_fp64": float(f"{r64['energy_drift']:.3e}")},
         "roofline": {
             "bound": "hbm",
-            "kernel": "one RK4 step (all kernels of the step; unit = 16 state vectors)",
+            "kernel": "k_stage (fused RK4 stage), mean of the 4 stage launches per step",
+            "bytes_per_launch": launch_bytes,
+            "us_per_launch": round(launch_us, 3),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -178,12 +217,12 @@ def main():
             "traffic": None,
         },
         "cpu_baseline": {
-            "value": round(cpu_val, 3),
+            "value": None if cpu_val is None else round(cpu_val, 3),
             "unit": "Mpts*steps/s",
-            "cores": 1,
+            "cores": cpu_threads,
             "kind": "port",
-            "sample": f"CPU oracle (scalar C, FP64), same N={N} sbp42 state, {args.cpu_steps} RK4 steps "
-                      f"({cpu_sec:.1f} s)",
+            "sample": f"CPU oracle (C, FP64, OpenMP row loops, {cpu_threads} threads), same N={N} "
+                      f"sbp42 state, {args.cpu_steps} RK4 steps ({cpu_sec:.1f} s)",
         },
     }
     print(json.dumps(line), flush=True)

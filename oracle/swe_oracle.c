@@ -16,6 +16,22 @@
 
 #define HALO 4 /* widest read across an edge: c2i SAT depth 3 */
 
+/* Row loops are split over OpenMP threads when built with -fopenmp (used for
+ * the timed CPU baseline).  Every parallel loop writes disjoint points and
+ * holds no reduction, so results do not depend on the thread count. */
+#ifdef _OPENMP
+#include <omp.h>
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#define OMP_FOR_COLLAPSE2 _Pragma("omp parallel for collapse(2) schedule(static)")
+int swo_num_threads(void) { return omp_get_max_threads(); }
+void swo_set_num_threads(int n) { omp_set_num_threads(n); }
+#else
+#define OMP_FOR
+#define OMP_FOR_COLLAPSE2
+int swo_num_threads(void) { return 1; }
+void swo_set_num_threads(int n) { (void)n; }
+#endif
+
 /* Planted mistakes for tests that prove the checks have power (0 = none):
  * 1 flip the sign of rotated Ch-vector halo entries, 2 drop the metric term
  * of the covariant edge sync, 3 read the depth-2 xy halo instead of the
@@ -60,6 +76,7 @@ static inline double *AT(const fld *f, int p, int i, int j)
 }
 static void fld_load(fld *f, const double *src)
 {
+    OMP_FOR_COLLAPSE2
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= f->ny; j++)
             for (int i = 1; i <= f->nx; i++)
@@ -67,6 +84,7 @@ static void fld_load(fld *f, const double *src)
 }
 static void fld_store(const fld *f, double *dst)
 {
+    OMP_FOR_COLLAPSE2
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= f->ny; j++)
             for (int i = 1; i <= f->nx; i++)
@@ -234,6 +252,7 @@ static void apply_map(const hmap *m, fld *a, fld *b)
 {
     fld *F[2] = {a, b};
     /* copy sources first: targets are halo points, sources interior */
+    OMP_FOR
     for (int k = 0; k < m->n; k++) {
         const hentry *e = &m->e[k];
         *AT(F[e->tc], e->tp, e->ti, e->tj) = e->sign * *AT(F[e->sc], e->sp, e->si, e->sj);
@@ -645,10 +664,12 @@ static void op_grad(swo *s)
     int N = s->N;
     if (g_mutation != 3) apply_map(&s->m_xy, &s->h, &s->h);
     for (int p = 0; p < 6; p++) {
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++) {
             d_i2c(s->order, AT(&s->h, p, 0, j), 1, N, AT(&s->gx, p, 0, j), 1);
             for (int i = 1; i <= N; i++) *AT(&s->gx, p, i, j) /= (s->hx * s->a);
         }
+        OMP_FOR
         for (int i = 1; i <= N + 1; i++) {
             d_i2c(s->order, AT(&s->h, p, i, 0), s->h.sx, N, AT(&s->gy, p, i, 0), s->gy.sx);
             for (int j = 1; j <= N; j++) *AT(&s->gy, p, i, j) /= (s->hx * s->a);
@@ -663,11 +684,14 @@ static void op_co2contra(swo *s)
     const geom *Gp = &s->G[SWO_MESH_XY], *Gu = &s->G[SWO_MESH_U], *Gv = &s->G[SWO_MESH_V];
     for (int p = 0; p < 6; p++) {
         /* uv -> xy (c2i): uq along x, vq along y */
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             i_c2i(s->order, AT(&s->u, p, 0, j), 1, N, AT(&s->w2, p, 0, j), 1);
+        OMP_FOR
         for (int i = 1; i <= N + 1; i++)
             i_c2i(s->order, AT(&s->v, p, i, 0), s->v.sx, N, AT(&s->w1, p, i, 0), s->w1.sx);
         /* at xy: w1 <- vq*J*Qi12 (feeds ut), w2 <- uq*J*Qi12 (feeds vt) */
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             for (int i = 1; i <= N + 1; i++) {
                 size_t id = PK(s, SWO_MESH_XY, p, i, j);
@@ -676,15 +700,19 @@ static void op_co2contra(swo *s)
                 *AT(&s->w2, p, i, j) *= c;
             }
         /* xy -> uv (i2c) and finalize */
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             i_i2c(s->order, AT(&s->w1, p, 0, j), 1, N, AT(&s->ut, p, 0, j), 1);
+        OMP_FOR
         for (int i = 1; i <= N + 1; i++)
             i_i2c(s->order, AT(&s->w2, p, i, 0), s->w2.sx, N, AT(&s->vt, p, i, 0), s->vt.sx);
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             for (int i = 1; i <= N; i++) {
                 size_t id = PK(s, SWO_MESH_U, p, i, j);
                 *AT(&s->ut, p, i, j) = Gu->Q11[id] * *AT(&s->u, p, i, j) + *AT(&s->ut, p, i, j) / Gu->J[id];
             }
+        OMP_FOR
         for (int j = 1; j <= N; j++)
             for (int i = 1; i <= N + 1; i++) {
                 size_t id = PK(s, SWO_MESH_V, p, i, j);
@@ -744,11 +772,14 @@ static void op_coriolis(swo *s, fld *cu, fld *cv)
     int N = s->N;
     const geom *Gu = &s->G[SWO_MESH_U], *Gv = &s->G[SWO_MESH_V];
     for (int p = 0; p < 6; p++) {
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             i_c2i(s->order, AT(&s->ut, p, 0, j), 1, N, AT(&s->w1, p, 0, j), 1);
+        OMP_FOR
         for (int i = 1; i <= N + 1; i++)
             i_c2i(s->order, AT(&s->vt, p, i, 0), s->vt.sx, N, AT(&s->w2, p, i, 0), s->w2.sx);
         /* calc_cori_at_p_tile L97-120: (up,vp) <- fJ2*(vp,-up) */
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             for (int i = 1; i <= N + 1; i++) {
                 double f = s->fJ2[PK(s, SWO_MESH_XY, p, i, j)];
@@ -759,10 +790,12 @@ static void op_coriolis(swo *s, fld *cu, fld *cv)
     apply_map(&s->m_xyv, &s->up, &s->vp);
     sync_covariant(s, &s->up, &s->vp);
     for (int p = 0; p < 6; p++) {
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++) {
             i_i2c(s->order, AT(&s->up, p, 0, j), 1, N, AT(cu, p, 0, j), 1);
             for (int i = 1; i <= N; i++) *AT(cu, p, i, j) /= Gu->J[PK(s, SWO_MESH_U, p, i, j)];
         }
+        OMP_FOR
         for (int i = 1; i <= N + 1; i++) {
             i_i2c(s->order, AT(&s->vp, p, i, 0), s->vp.sx, N, AT(cv, p, i, 0), cv->sx);
             for (int j = 1; j <= N; j++) *AT(cv, p, i, j) /= Gv->J[PK(s, SWO_MESH_V, p, i, j)];
@@ -800,19 +833,24 @@ static void op_div(swo *s)
     int N = s->N;
     const geom *Gp = &s->G[SWO_MESH_XY], *Gu = &s->G[SWO_MESH_U], *Gv = &s->G[SWO_MESH_V];
     for (int p = 0; p < 6; p++) {
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             for (int i = 1; i <= N; i++)
                 *AT(&s->Ju, p, i, j) = *AT(&s->ut, p, i, j) * Gu->J[PK(s, SWO_MESH_U, p, i, j)];
+        OMP_FOR
         for (int j = 1; j <= N; j++)
             for (int i = 1; i <= N + 1; i++)
                 *AT(&s->Jv, p, i, j) = *AT(&s->vt, p, i, j) * Gv->J[PK(s, SWO_MESH_V, p, i, j)];
     }
     apply_map(&s->m_ch, &s->Ju, &s->Jv);
     for (int p = 0; p < 6; p++) {
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             d_c2i(s->order, AT(&s->Ju, p, 0, j), 1, N, AT(&s->Dx, p, 0, j), 1);
+        OMP_FOR
         for (int i = 1; i <= N + 1; i++)
             d_c2i(s->order, AT(&s->Jv, p, i, 0), s->Jv.sx, N, AT(&s->Dy, p, i, 0), s->Dy.sx);
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             for (int i = 1; i <= N + 1; i++)
                 *AT(&s->div, p, i, j) = (*AT(&s->Dx, p, i, j) / s->hx + *AT(&s->Dy, p, i, j) / s->hx) /
@@ -835,12 +873,15 @@ static void apply_loaded(swo *s, double *du, double *dv, double *dh)
     op_coriolis(s, &cu, &cv);
     op_div(s);
     for (int p = 0; p < 6; p++) {
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             for (int i = 1; i <= N; i++)
                 du[PK(s, SWO_MESH_U, p, i, j)] = -s->g * *AT(&s->gx, p, i, j) + *AT(&cu, p, i, j);
+        OMP_FOR
         for (int j = 1; j <= N; j++)
             for (int i = 1; i <= N + 1; i++)
                 dv[PK(s, SWO_MESH_V, p, i, j)] = -s->g * *AT(&s->gy, p, i, j) + *AT(&cv, p, i, j);
+        OMP_FOR
         for (int j = 1; j <= N + 1; j++)
             for (int i = 1; i <= N + 1; i++)
                 dh[PK(s, SWO_MESH_XY, p, i, j)] = -s->H0 * *AT(&s->div, p, i, j);
@@ -906,15 +947,19 @@ void swo_rk4(swo *s, double *u, double *v, double *h, double dt, int nsteps)
     for (int it = 0; it < nsteps; it++) {
         swo_apply(s, v0[0], v0[1], v0[2], s->k[0][0], s->k[0][1], s->k[0][2]);
         for (int c = 0; c < 3; c++)
+            OMP_FOR
             for (int n = 0; n < n3[c]; n++) s->y[c][n] = 1.0 * v0[c][n] + 0.5 * dt * s->k[0][c][n];
         swo_apply(s, s->y[0], s->y[1], s->y[2], s->k[1][0], s->k[1][1], s->k[1][2]);
         for (int c = 0; c < 3; c++)
+            OMP_FOR
             for (int n = 0; n < n3[c]; n++) s->y[c][n] = 1.0 * v0[c][n] + 0.5 * dt * s->k[1][c][n];
         swo_apply(s, s->y[0], s->y[1], s->y[2], s->k[2][0], s->k[2][1], s->k[2][2]);
         for (int c = 0; c < 3; c++)
+            OMP_FOR
             for (int n = 0; n < n3[c]; n++) s->y[c][n] = 1.0 * v0[c][n] + dt * s->k[2][c][n];
         swo_apply(s, s->y[0], s->y[1], s->y[2], s->k[3][0], s->k[3][1], s->k[3][2]);
         for (int c = 0; c < 3; c++)
+            OMP_FOR
             for (int n = 0; n < n3[c]; n++) {
                 v0[c][n] = v0[c][n] + dt / 6.0 * s->k[0][c][n] + dt / 3.0 * s->k[1][c][n];
                 v0[c][n] = v0[c][n] + dt / 3.0 * s->k[2][c][n] + dt / 6.0 * s->k[3][c][n];

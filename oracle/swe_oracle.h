@@ -84,6 +84,10 @@ void swo_line_op(int order, int op, int n, const double *in, double *out, int wi
 void swo_basis(const swo *s, int mesh, double *a1, double *a2);
 void swo_set_mutation(int m);
 
+/* OpenMP threads used by the row loops (1 when built without OpenMP). */
+int swo_num_threads(void);
+void swo_set_num_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

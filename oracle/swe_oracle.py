@@ -59,6 +59,7 @@ def _load():
     lib.swo_line_op.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]
     lib.swo_basis.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
     lib.swo_set_mutation.argtypes = [ctypes.c_int]
+    lib.swo_set_num_threads.argtypes = [ctypes.c_int]
     _lib = lib
     return lib
 
@@ -80,6 +81,15 @@ def line_op(order, op, n, f, with_sat=True):
     out = np.zeros(nout)
     lib.swo_line_op(order, op, n, _p(f), _p(out), 1 if with_sat else 0)
     return out
+
+
+def num_threads():
+    """OpenMP threads the oracle's row loops use."""
+    return _load().swo_num_threads()
+
+
+def set_num_threads(n):
+    _load().swo_set_num_threads(int(n))
 
 
 def set_mutation(m):
