@@ -25,7 +25,7 @@ import numpy as np
 
 __all__ = [
     "SweError", "lib", "Config", "OperatorSwmLin", "StvecSwm", "Rk4",
-    "create_linear_swm_operator", "create_timescheme", "halo_map",
+    "create_linear_swm_operator", "create_timescheme", "halo_map", "Ensemble",
     "FP64", "FP32", "LIB_PATH",
 ]
 
@@ -38,7 +38,8 @@ EXPORTS = (
     "swe_last_error", "swe_version", "swe_plan_create", "swe_plan_destroy", "swe_plan_npts",
     "swe_plan_stream", "swe_synchronize", "swe_state_create", "swe_state_destroy",
     "swe_state_upload", "swe_state_download", "swe_apply", "swe_assign", "swe_update",
-    "swe_rk4_steps", "swe_integrals", "swe_halo_map",
+    "swe_rk4_steps", "swe_integrals", "swe_halo_map", "swe_ensemble_create",
+    "swe_ensemble_destroy", "swe_ensemble_rk4", "swe_ensemble_stream", "swe_ensemble_synchronize",
 )
 
 
@@ -87,6 +88,12 @@ def lib():
     L.swe_rk4_steps.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_int]
     L.swe_integrals.argtypes = [_vp, _vp, _dp, _dp]
     L.swe_halo_map.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip] + [_ip] * 9
+    L.swe_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
+    L.swe_ensemble_destroy.argtypes = [_vp]
+    L.swe_ensemble_rk4.argtypes = [_vp, _dp, _ip, ctypes.c_int]
+    L.swe_ensemble_stream.argtypes = [_vp]
+    L.swe_ensemble_stream.restype = _vp
+    L.swe_ensemble_synchronize.argtypes = [_vp]
     _lib = L
     return L
 
@@ -101,6 +108,11 @@ def _p(a):
     return a.ctypes.data_as(_dp)
 
 
+# swe_config.flags and swe_ensemble_rk4 flags (include/ch_swe.h)
+FLAG_UNFUSED, FLAG_LINEAR_BLOCKS = 1, 2
+ENS_GRAPH = 1
+
+
 class Config:
     """Operator configuration (the reference's domain/metric/operator config
     keys: N, planet_radius, omega, rotation_axis, coriolis_constant, h_mean,
@@ -108,9 +120,10 @@ class Config:
 
     def __init__(self, N, order=42, precision=FP64, radius=6.371e6, grav=9.81,
                  h_mean=1000.0, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
-                 coriolis_const=False, unfused=False):
+                 coriolis_const=False, unfused=False, linear_blocks=False):
         self.N, self.order, self.precision = int(N), int(order), int(precision)
         self.unfused = bool(unfused)
+        self.linear_blocks = bool(linear_blocks)
         self.radius, self.grav, self.h_mean, self.omega = radius, grav, h_mean, omega
         self.axis = tuple(float(a) for a in axis)
         self.coriolis_const = bool(coriolis_const)
@@ -121,7 +134,7 @@ class Config:
         c.coriolis_const = 1 if self.coriolis_const else 0
         c.radius, c.grav, c.h_mean, c.omega = self.radius, self.grav, self.h_mean, self.omega
         c.axis[:] = self.axis
-        c.flags = 1 if self.unfused else 0
+        c.flags = (FLAG_UNFUSED if self.unfused else 0) | (FLAG_LINEAR_BLOCKS if self.linear_blocks else 0)
         return c
 
 
@@ -221,12 +234,50 @@ class Rk4:
 
 def create_linear_swm_operator(N, order=42, precision=FP64, grav=9.81, h_mean=1000.0,
                                radius=6.371e6, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
-                               coriolis_const=False, unfused=False):
+                               coriolis_const=False, unfused=False, linear_blocks=False):
     """Factory mirroring create_linear_swm_operator
     (operator_swm_factory_mod.f90 L586-685) for the Ch staggering.
-    unfused=True selects the multi-kernel diagnostic path."""
+    unfused=True selects the multi-kernel diagnostic path; linear_blocks=True
+    the plain block order of the fused stage kernel."""
     return OperatorSwmLin(Config(N, order, precision, radius, grav, h_mean, omega, axis,
-                                 coriolis_const, unfused))
+                                 coriolis_const, unfused, linear_blocks))
+
+
+class Ensemble:
+    """Many independent solves advanced together (swe_ensemble_*): each RK4
+    stage of every member is one launch of the fused stage kernel.  members
+    are StvecSwm states of fused plans of one precision (plans may differ in
+    N and order, members may share a plan)."""
+
+    def __init__(self, members):
+        self.members = list(members)  # keeps the states and their plans alive
+        arr = (_vp * len(self.members))(*[m.h for m in self.members])
+        h = _vp()
+        _check(lib().swe_ensemble_create(len(self.members), arr, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().swe_ensemble_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        self.close()
+
+    def rk4(self, dt, nsteps, graph=False):
+        """Advance member i by nsteps[i] RK4 steps of dt[i] (scalars broadcast);
+        asynchronous on the ensemble's stream, ordered with the plan streams."""
+        n = len(self.members)
+        dts = np.broadcast_to(np.asarray(dt, dtype=np.float64), (n,)).copy()
+        ns = np.broadcast_to(np.asarray(nsteps, dtype=np.int32), (n,)).copy()
+        _check(lib().swe_ensemble_rk4(self.h, _p(dts), ns.ctypes.data_as(_ip), ENS_GRAPH if graph else 0))
+
+    def synchronize(self):
+        _check(lib().swe_ensemble_synchronize(self.h))
+
+    @property
+    def stream(self):
+        return lib().swe_ensemble_stream(self.h)
 
 
 def create_timescheme(name):

@@ -51,9 +51,34 @@ def build_fortran(verbose=False):
     return out
 
 
+def _asm_cmd(out):
+    """Device-only assembly of the kernels, for the ISA check."""
+    flags = [f for f in FLAGS if f not in ("-shared", "-Wall", "-Wno-unused-function")]
+    return [HIPCC] + flags + ["--cuda-device-only", "-S", "-w", os.path.join(CSRC, "swe_lib.hip"), "-o", out]
+
+
 def build(verbose=False):
     cmd = [HIPCC] + FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", LIB + ".tmp"]
-    _run(cmd, "hipcc (libchswe.so)", verbose)
+    asm_dir = os.path.join(HERE, "_build")
+    os.makedirs(asm_dir, exist_ok=True)
+    asm = os.path.join(asm_dir, "swe_lib_gfx950.s")
+    asm_proc = subprocess.Popen(_asm_cmd(asm), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    try:
+        _run(cmd, "hipcc (libchswe.so)", verbose)
+    finally:
+        asm_out, _ = asm_proc.communicate()
+    if asm_proc.returncode != 0:
+        sys.stderr.write(asm_out)
+        raise RuntimeError("hipcc -S (ISA check) failed")
+    if HERE not in sys.path:
+        sys.path.insert(0, HERE)
+    import isa_check
+
+    found = isa_check.check_file(asm)
+    if found:
+        lines = [f"{k}: {len(v)} address reads of SGPRs undefined on some path (first at line {v[0][0]})"
+                 for k, v in found.items()]
+        raise RuntimeError("ISA check failed:\n" + "\n".join(lines))
     os.replace(LIB + ".tmp", LIB)
     build_fortran(verbose)
     return LIB

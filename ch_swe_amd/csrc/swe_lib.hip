@@ -50,151 +50,7 @@ int fail(int code, const std::string &msg)
             return fail(SWE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));             \
     } while (0)
 
-constexpr int HALO = 4;
-constexpr double PI = 3.14159265358979323846;
-
-// ------------------------------------------------------------------------
-// layout
-// ------------------------------------------------------------------------
-struct Dims {
-    int N;     // cells per panel edge
-    int P;     // row pitch (elements)
-    long PS;   // panel stride (elements)
-    long F;    // field stride = 6 * PS
-};
-
-Dims make_dims(int N)
-{
-    Dims d;
-    d.N = N;
-    d.P = ((N + 1 + 2 * HALO) + 15) / 16 * 16;
-    d.PS = (long)d.P * (N + 1 + 2 * HALO);
-    d.F = 6 * d.PS;
-    return d;
-}
-
-__host__ __device__ inline long off(const Dims &d, int p, int i, int j)
-{
-    return p * d.PS + (long)(j - 1 + HALO) * d.P + (i - 1 + HALO);
-}
-
-// flat interior index -> (p, i, j) for a mesh with nx x ny points per panel
-__device__ inline void unflat(long n, int nx, int ny, int &p, int &i, int &j)
-{
-    long per = (long)nx * ny;
-    p = (int)(n / per);
-    long r = n - p * per;
-    j = (int)(r / nx) + 1;
-    i = (int)(r - (long)(j - 1) * nx) + 1;
-}
-
-// ------------------------------------------------------------------------
-// geometry (FP64 on the device): metric/metric_2d_ecs_mod.f90 L338-411,
-// mesh/mesh_factory_mod.f90 L36-37, coriolis_factory_mod.f90 L239-280,
-// quadrature_factory_mod.f90 L89-188
-// ------------------------------------------------------------------------
-__constant__ int c_EX[6][3] = {{-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 0, 1}, {1, 0, 0}, {1, 0, 0}};
-__constant__ int c_EY[6][3] = {{0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, -1}};
-__constant__ int c_NN[6][3] = {{0, 0, 1}, {-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
-
-struct GeomParams {
-    double hx, radius, omega, axis[3];
-    int coriolis_const, order;
-};
-
-__device__ inline double sbp_weight(int idx, int n, bool centre, int order)
-{
-    // D21_A_interfaces/centers, D42_A_interfaces/centers
-    // (sbp_operators_collection_mod.f90 L169-170, L196-197)
-    const double i21[2] = {0.5, 1.0}, c21[2] = {1.0, 1.0};
-    const double i42[5] = {7.0 / 18, 9.0 / 8, 1.0, 71.0 / 72, 1.0};
-    const double c42[4] = {13.0 / 12, 7.0 / 8, 25.0 / 24, 1.0};
-    const double *A = order == 21 ? (centre ? c21 : i21) : (centre ? c42 : i42);
-    int ne = order == 21 ? 2 : (centre ? 4 : 5);
-    double w = 1.0;
-    if (idx <= ne) w = A[idx - 1];
-    if (n - idx + 1 <= ne) w = A[n - idx];
-    return w;
-}
-
-// mesh m: 0 xy, 1 u (x centres), 2 v (y centres).  Writes FP64 J, Qi, f.
-template <typename T>
-__global__ void k_metric(Dims d, GeomParams g, int m, T *c0, T *c1, T *c2, double *w)
-{
-    const int N = d.N;
-    const int xt = m == 1, yt = m == 2;
-    const int nx = xt ? N : N + 1, ny = yt ? N : N + 1;
-    long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (n >= 6L * nx * ny) return;
-    int p, i, j;
-    unflat(n, nx, ny, p, i, j);
-    double al = -0.25 * PI + (i - 1 + 0.5 * xt) * g.hx;
-    double be = -0.25 * PI + (j - 1 + 0.5 * yt) * g.hx;
-    double X = tan(al), Y = tan(be);
-    double s2 = 1.0 + X * X + Y * Y, s = sqrt(s2);
-    double J = (1.0 + X * X) * (1.0 + Y * Y) / (s2 * s);
-    double q11 = s2 / (1.0 + X * X), q12 = s2 * X * Y / ((1.0 + X * X) * (1.0 + Y * Y)),
-           q22 = s2 / (1.0 + Y * Y);
-    long o = off(d, p, i, j);
-    w[o] = J * g.hx * g.hx * sbp_weight(i, nx, xt, g.order) * sbp_weight(j, ny, yt, g.order);
-    if (m == 0) {
-        // r = ex X/s + ey Y/s - n 1/s (ecs_proto2realface with identity rotation)
-        double r[3];
-        for (int k = 0; k < 3; k++) r[k] = (c_EX[p][k] * X + c_EY[p][k] * Y - c_NN[p][k]) / s;
-        double f;
-        if (g.coriolis_const) {
-            f = g.omega;
-        } else {
-            double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-            f = 2.0 * g.omega * (g.axis[0] * r[0] + g.axis[1] * r[1] + g.axis[2] * r[2]) / rn;
-        }
-        c0[o] = T(J * q12);              // J Qi12 (co2contra cross term)
-        c1[o] = T(f * J * J);            // f J^2 (Coriolis)
-        c2[o] = T(1.0 / (J * g.radius * g.hx));  // 1/(hx J a) (div)
-    } else {
-        c0[o] = T(m == 1 ? q11 : q22);   // Qi11 at u, Qi22 at v
-        c1[o] = T(1.0 / J);
-        c2[o] = T(J);
-    }
-}
-
-// covariant edge-sync weights (ecs_Ah_vec_sync_factory_mod.f90 L97-152):
-// own across-edge covariant vector . neighbour's along-edge contravariant
-// vector, the neighbour's prototype face folded onto this one.
-__device__ inline void ecs_basis(double al, double be, double a1[3], double a2[3], double b1[3], double b2[3])
-{
-    double X = tan(al), Y = tan(be);
-    double s2 = 1 + X * X + Y * Y, s3 = s2 * sqrt(s2);
-    double cx = 1 + X * X, cy = 1 + Y * Y;
-    a1[0] = cx * cy / s3; a1[1] = -X * Y * cx / s3; a1[2] = -X * cx / s3;
-    a2[0] = -X * Y * cy / s3; a2[1] = cx * cy / s3; a2[2] = -Y * cy / s3;
-    double q11 = s2 / cx, q12 = s2 * X * Y / (cx * cy), q22 = s2 / cy;
-    for (int k = 0; k < 3; k++) {
-        b1[k] = q11 * a1[k] + q12 * a2[k];
-        b2[k] = q12 * a1[k] + q22 * a2[k];
-    }
-}
-
-template <typename T>
-__global__ void k_sync_weights(int N, double hx, T *qb, T *qt, T *ql, T *qr)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    if (i > N + 1) return;
-    double c = -0.25 * PI + (i - 1) * hx, e = 0.25 * PI;
-    double a1[3], a2[3], b1[3], b2[3], A1[3], A2[3], B1[3], B2[3];
-    ecs_basis(c, -e, a1, a2, b1, b2);
-    ecs_basis(c, e, A1, A2, B1, B2);
-    qb[i - 1] = T(a2[0] * B1[0] - a2[1] * B1[2] + a2[2] * B1[1]);
-    ecs_basis(c, e, a1, a2, b1, b2);
-    ecs_basis(c, -e, A1, A2, B1, B2);
-    qt[i - 1] = T(a2[0] * B1[0] + a2[1] * B1[2] - a2[2] * B1[1]);
-    ecs_basis(-e, c, a1, a2, b1, b2);
-    ecs_basis(e, c, A1, A2, B1, B2);
-    ql[i - 1] = T(-a1[0] * B2[2] + a1[1] * B2[1] + a1[2] * B2[0]);
-    ecs_basis(e, c, a1, a2, b1, b2);
-    ecs_basis(-e, c, A1, A2, B1, B2);
-    qr[i - 1] = T(a1[0] * B2[2] + a1[1] * B2[1] - a1[2] * B2[0]);
-}
+#include "swe_geom.hpp"
 
 // ------------------------------------------------------------------------
 // operator kernels
@@ -442,58 +298,7 @@ __global__ void k_reduce_final(const double *part, int nb, double *out)
 }
 
 #include "swe_fused.inc"
-
-// panel-independent coefficient tables for the fused path (FP64 arithmetic,
-// stored in T) on the (N+1) x (N+1) index grid, plus f J^2 per panel
-template <typename T>
-__global__ void k_coef(int N, GeomParams g, T *JQ12, T *iJa, T *Q11, T *iJu, T *Ju, T *Q22, T *iJv, T *Jv, T *fJ2)
-{
-    const int n = N + 1;
-    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (k >= 6L * n * n) return;
-    int p = (int)(k / ((long)n * n));
-    int r = (int)(k - (long)p * n * n);
-    int j = r / n + 1, i = r % n + 1;
-    auto metric = [&](double xs, double ys, double &J, double &q11, double &q12, double &q22, double &X,
-                      double &Y, double &s) {
-        double al = -0.25 * PI + (i - 1 + xs) * g.hx, be = -0.25 * PI + (j - 1 + ys) * g.hx;
-        X = tan(al);
-        Y = tan(be);
-        double s2 = 1.0 + X * X + Y * Y;
-        s = sqrt(s2);
-        J = (1.0 + X * X) * (1.0 + Y * Y) / (s2 * s);
-        q11 = s2 / (1.0 + X * X);
-        q12 = s2 * X * Y / ((1.0 + X * X) * (1.0 + Y * Y));
-        q22 = s2 / (1.0 + Y * Y);
-    };
-    double J, q11, q12, q22, X, Y, s;
-    metric(0.0, 0.0, J, q11, q12, q22, X, Y, s);
-    double f;
-    if (g.coriolis_const) {
-        f = g.omega;
-    } else {
-        double rr[3];
-        for (int m = 0; m < 3; m++) rr[m] = (c_EX[p][m] * X + c_EY[p][m] * Y - c_NN[p][m]) / s;
-        double rn = sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
-        f = 2.0 * g.omega * (g.axis[0] * rr[0] + g.axis[1] * rr[1] + g.axis[2] * rr[2]) / rn;
-    }
-    fJ2[k] = T(f * J * J);
-    if (p != 0) return;
-    JQ12[r] = T(J * q12);
-    iJa[r] = T(1.0 / (J * g.radius * g.hx));
-    if (i <= N) {
-        metric(0.5, 0.0, J, q11, q12, q22, X, Y, s);
-        Q11[r] = T(q11);
-        iJu[r] = T(1.0 / J);
-        Ju[r] = T(J);
-    }
-    if (j <= N) {
-        metric(0.0, 0.5, J, q11, q12, q22, X, Y, s);
-        Q22[r] = T(q22);
-        iJv[r] = T(1.0 / J);
-        Jv[r] = T(J);
-    }
-}
+#include "swe_fused_host.hpp"
 
 // ------------------------------------------------------------------------
 // engine
@@ -639,7 +444,7 @@ struct Engine : EngineBase {
         if ((rc = upload_map(0, 1, {0, 0}, &maps.xy, &maps.n_xy))) return rc;
         if ((rc = upload_map(1, order == 21 ? 2 : 3, {0, 1}, &maps.ch, &maps.n_ch))) return rc;
         if ((rc = upload_map(2, 1, {0, 1}, &maps.xyv, &maps.n_xyv))) return rc;
-        use_fused = !(p->cfg.flags & 1);
+        use_fused = !(p->cfg.flags & SWE_FLAG_UNFUSED);
         if ((rc = init_fused(g))) return rc;
         HIP_TRY(hipStreamSynchronize(pl->stream));
         return SWE_OK;
@@ -670,63 +475,15 @@ struct Engine : EngineBase {
             }
         HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_side), sd, sizeof(sd)));
         // tiles, edge segments and which tiles write into each segment's band
-        ntx = (n + F_TX - 1) / F_TX;
-        nty = (n + TY - 1) / TY;
-        ntiles = 6 * ntx * nty;
-        auto ext = [n](int lo, int hi, int &a, int &b) {  // ext_rg for region 0 (m = 14)
-            for (int pass = 0; pass < 2; pass++) {
-                lo = lo <= 3 ? 1 : lo;
-                hi = hi >= n - 2 ? n : hi;
-                if (hi >= n) lo = std::min(lo, n - 13);
-                if (lo <= 1) hi = std::max(hi, 14);
-            }
-            a = std::max(lo, 1);
-            b = std::min(hi, n);
-        };
-        struct Box { int x0, x1, y0, y1; };
-        std::vector<Box> tiles(ntiles);
-        for (int p = 0; p < 6; p++)
-            for (int ty = 0; ty < nty; ty++)
-                for (int tx = 0; tx < ntx; tx++) {
-                    int i0 = 1 + tx * F_TX, j0 = 1 + ty * TY;
-                    tiles[(p * nty + ty) * ntx + tx] = Box{i0, std::min(n, i0 + F_TX - 1), j0, std::min(n, j0 + TY - 1)};
-                }
-        std::vector<int4> segs;
-        std::vector<Box> bands;
-        std::vector<int> segp;
-        for (int p = 0; p < 6; p++)
-            for (int side = 0; side < 4; side++) {
-                int cnt = side <= 1 ? nty : ntx;
-                for (int k = 0; k < cnt; k++) {
-                    int lo = side <= 1 ? 1 + k * TY : 1 + k * F_TX;
-                    int hi = std::min(n, side <= 1 ? lo + TY - 1 : lo + F_TX - 1);
-                    segs.push_back(make_int4(p, side, lo, hi));
-                    Box b;
-                    int bx0, bx1, by0, by1;
-                    if (side == 0) { bx0 = 1; bx1 = 4; by0 = lo; by1 = hi; }
-                    else if (side == 1) { bx0 = n - 3; bx1 = n; by0 = lo; by1 = hi; }
-                    else if (side == 2) { bx0 = lo; bx1 = hi; by0 = 1; by1 = 4; }
-                    else { bx0 = lo; bx1 = hi; by0 = n - 3; by1 = n; }
-                    ext(bx0 - 6, bx1 + 6, b.x0, b.x1);
-                    ext(by0 - 6, by1 + 6, b.y0, b.y1);
-                    bands.push_back(b);
-                    segp.push_back(p);
-                }
-            }
-        nseg = (int)segs.size();
-        std::vector<int> ts((size_t)ntiles * F_MAXSEG, -1), expect(nseg, 0);
-        for (int t = 0; t < ntiles; t++) {
-            int p = t / (ntx * nty), cnt = 0;
-            const Box &a = tiles[t];
-            for (int k = 0; k < nseg; k++) {
-                if (segp[k] != p) continue;
-                const Box &b = bands[k];
-                if (a.x1 < b.x0 || b.x1 < a.x0 || a.y1 < b.y0 || b.y1 < a.y0) continue;
-                if (cnt == F_MAXSEG) return fail(SWE_EINVAL, "internal: tile touches too many edge segments");
-                ts[(size_t)t * F_MAXSEG + cnt++] = k;
-                expect[k]++;
-            }
-        }
+        FusedTables ft;
+        std::string err;
+        if (build_fused_tables(N, TY, ft, err)) return fail(SWE_EINVAL, err);
+        ntx = ft.ntx;
+        nty = ft.nty;
+        ntiles = ft.ntiles;
+        nseg = ft.nseg;
+        const std::vector<int> &ts = ft.tile_segs, &expect = ft.expect;
+        const std::vector<int4> &segs = ft.segs;
         if ((rc = alloc((void **)&tile_segs, ts.size() * sizeof(int)))) return rc;
         if ((rc = alloc((void **)&seg_expect, expect.size() * sizeof(int)))) return rc;
         if ((rc = alloc((void **)&seg_cnt, expect.size() * sizeof(int)))) return rc;
@@ -755,6 +512,8 @@ struct Engine : EngineBase {
         a.seg_expect = seg_expect;
         a.seg_desc = seg_desc;
         a.seg_cnt = seg_cnt;
+        a.order = O;
+        a.xcd = (pl->cfg.flags & SWE_FLAG_LINEAR_BLOCKS) ? 0 : 1;
         return a;
     }
 
@@ -770,10 +529,8 @@ struct Engine : EngineBase {
         launch_records<O>(in, rec[0]);
         StageArgs<T> a = stage_args<O>(in);
         a.out = out;
-        a.mode = 0;
         a.rec_in = rec[0];
-        a.produce = 0;
-        k_stage<T, O, TY><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
+        k_stage<T, O, TY, 0><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
     }
 
     template <int O>
@@ -793,11 +550,11 @@ struct Engine : EngineBase {
                 a.out = v0;
                 a.a_dt = T(A[st]);
                 a.b_dt = T(B[st]);
-                a.mode = st == 0 ? 1 : (st == 3 ? 3 : 2);
                 a.rec_in = rec[st & 1];
                 a.rec_out = rec[(st + 1) & 1];
-                a.produce = 1;
-                k_stage<T, O, TY><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
+                if (st == 0) k_stage<T, O, TY, 1><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
+                else if (st == 3) k_stage<T, O, TY, 3><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
+                else k_stage<T, O, TY, 2><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
             }
     }
 
@@ -1141,3 +898,4 @@ int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *
 }
 
 }  // extern "C"
+#include "swe_ensemble.inc"

@@ -33,6 +33,15 @@ enum {
 
 enum { SWE_FP64 = 0, SWE_FP32 = 1 };
 
+/* swe_config.flags */
+enum {
+    SWE_FLAG_UNFUSED = 1,      /* run the unfused multi-kernel path (diagnostics) */
+    SWE_FLAG_LINEAR_BLOCKS = 2 /* plain block order instead of the XCD-aware one */
+};
+
+/* swe_ensemble_rk4 flags */
+enum { SWE_ENS_GRAPH = 1 /* replay the step loop from captured HIP graphs */ };
+
 typedef struct swe_plan swe_plan;
 typedef struct swe_state swe_state;
 
@@ -51,7 +60,7 @@ typedef struct {
     double h_mean;      /* mean depth H0 [m] */
     double omega;       /* rotation rate [1/s] */
     double axis[3];     /* rotation axis */
-    int flags;          /* bit 0: run the unfused multi-kernel path (diagnostics) */
+    int flags;          /* SWE_FLAG_* bits */
 } swe_config;
 
 const char *swe_last_error(void);
@@ -105,6 +114,28 @@ int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy);
  * query the count. */
 int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *tj, int *tc,
                  int *sp, int *si, int *sj, int *sc, int *sign);
+
+/* Ensembles and uneven batches: many independent solves advanced together,
+ * every RK4 stage of all members in one launch of the fused stage kernel
+ * (a work list of (member, tile) items).  No counterpart in the reference,
+ * which runs one solve per program; each member's arithmetic is exactly that
+ * of swe_rk4_steps on its own plan, so results do not depend on the mix.
+ *   members : distinct states; their plans may differ in N and SBP order but
+ *             share one precision and use the fused path; the plans must
+ *             outlive the ensemble.  Members may share a plan.
+ * The ensemble owns each member's RK4 registers and edge records and one
+ * stream (default priority). */
+typedef struct swe_ensemble swe_ensemble;
+int swe_ensemble_create(int n, swe_state *const *members, swe_ensemble **out);
+int swe_ensemble_destroy(swe_ensemble *ens);
+/* Advance member i by nsteps[i] >= 0 RK4 steps of size dt[i].  Asynchronous
+ * on the ensemble's stream, ordered after earlier work on the members' plan
+ * streams and before later work on them (stream events), so
+ * swe_state_download / swe_integrals afterwards see the result.  flags:
+ * SWE_ENS_GRAPH. */
+int swe_ensemble_rk4(swe_ensemble *ens, const double *dt, const int *nsteps, int flags);
+void *swe_ensemble_stream(swe_ensemble *ens);
+int swe_ensemble_synchronize(swe_ensemble *ens);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,107 @@
+"""GPU: ensembles and uneven batches (swe_ensemble_*).  Every member of a
+mixed batch must give what the same solve gives alone (swe_rk4_steps on its
+own plan) and, in FP64, what the oracle gives (1e-12 relative)."""
+import sys
+
+import numpy as np
+import pytest
+
+import ch_swe_amd as cs
+from oracle.swe_oracle import MESH_U, MESH_V, MESH_XY
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _state(N, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((6, N + 1, N)), rng.standard_normal((6, N, N + 1)),
+            rng.standard_normal((6, N + 1, N + 1)))
+
+
+def _alone(N, order, precision, state0, dt, nsteps):
+    op = cs.create_linear_swm_operator(N, order, precision)
+    st = op.create_stvec().upload(*state0)
+    cs.create_timescheme("rk4").step(st, op, dt, nsteps)
+    return st.download()
+
+
+# (N, order, dt, nsteps): different sizes, both orders, own dt and step
+# counts, one member that does not move, two members on one plan
+MIXED = [(16, 42, 900.0, 7), (12, 21, 1200.0, 3), (16, 42, 600.0, 0), (20, 42, 700.0, 5),
+         (9, 21, 1500.0, 6), (16, 42, 450.0, 4)]
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_mixed_fp64_members_match_alone_and_oracle(oracle_mod, graph):
+    _say(f"ensemble fp64 mixed graph={graph}")
+    plans = {}
+    members, states0 = [], []
+    for k, (N, order, dt, ns) in enumerate(MIXED):
+        key = (N, order)
+        if key not in plans:
+            plans[key] = cs.create_linear_swm_operator(N, order, cs.FP64)
+        s0 = _state(N, 100 + k)
+        states0.append(s0)
+        members.append(plans[key].create_stvec().upload(*s0))
+    ens = cs.Ensemble(members)
+    ens.rk4([m[2] for m in MIXED], [m[3] for m in MIXED], graph=graph)
+    for k, (N, order, dt, ns) in enumerate(MIXED):
+        got = members[k].download()
+        alone = _alone(N, order, cs.FP64, states0[k], dt, ns)
+        ref = oracle_mod.Oracle(N, order).rk4(*states0[k], dt, ns) if ns else states0[k]
+        for g, a, r in zip(got, alone, ref):
+            assert _rel(g, a) < 1e-14, (k, N, order)
+            assert _rel(g, r) < 1e-12, (k, N, order)
+
+
+def test_uniform_fp32_graph_replay_and_second_run():
+    """Members on one plan, a step count past one captured graph (64 steps),
+    then a second, shorter run on the same ensemble (counters and records
+    must be reusable)."""
+    _say("ensemble fp32 uniform graph")
+    N, order, dt = 24, 42, 800.0
+    op = cs.create_linear_swm_operator(N, order, cs.FP32)
+    s0 = [_state(N, 300 + k) for k in range(5)]
+    members = [op.create_stvec().upload(*s) for s in s0]
+    ens = cs.Ensemble(members)
+    ens.rk4(dt, 70, graph=True)
+    ens.rk4(dt, [3, 1, 0, 2, 3], graph=True)
+    extra = [3, 1, 0, 2, 3]
+    for k in range(5):
+        got = members[k].download()
+        op2 = cs.create_linear_swm_operator(N, order, cs.FP32)
+        st = op2.create_stvec().upload(*s0[k])
+        rk = cs.create_timescheme("rk4")
+        rk.step(st, op2, dt, 70)
+        if extra[k]:
+            rk.step(st, op2, dt, extra[k])
+        for g, a in zip(got, st.download()):
+            assert _rel(g, a) < 1e-6, k
+
+
+def test_ensemble_rejects_bad_members():
+    _say("ensemble errors")
+    a = cs.create_linear_swm_operator(12, 42, cs.FP64)
+    b = cs.create_linear_swm_operator(12, 42, cs.FP32)
+    c = cs.create_linear_swm_operator(12, 42, cs.FP64, unfused=True)
+    sa, sb, sc = a.create_stvec(), b.create_stvec(), c.create_stvec()
+    with pytest.raises(cs.SweError, match="precision"):
+        cs.Ensemble([sa, sb])
+    with pytest.raises(cs.SweError, match="fused"):
+        cs.Ensemble([sa, sc])
+    with pytest.raises(cs.SweError, match="repeated"):
+        cs.Ensemble([sa, sa])
+    ens = cs.Ensemble([sa])
+    with pytest.raises(cs.SweError, match="nsteps"):
+        ens.rk4(100.0, -1)
+    with pytest.raises(cs.SweError, match="finite"):
+        ens.rk4(float("nan"), 1)

@@ -1,0 +1,58 @@
+"""CPU: the build-time ISA check flags an address computation that reads an
+SGPR left undefined on one path, and passes the same code once every path
+defines it."""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "ch_swe_amd"))
+import isa_check  # noqa: E402
+
+# two paths into .LBB0_3: the first writes s[28:29], the second leaves it as
+# an implicit-def; .LBB0_3 stores through it
+BAD = """
+_Z4kernv: ; @_Z4kernv
+; %bb.0:
+\ts_load_dwordx4 s[8:11], s[0:1], 0x0
+\ts_cmp_eq_u32 s2, 3
+\ts_cbranch_scc1 .LBB0_2
+; %bb.1:
+\ts_mov_b64 s[28:29], s[8:9]
+\ts_branch .LBB0_3
+.LBB0_2:
+                                        ; implicit-def: $sgpr28_sgpr29
+\ts_nop 0
+.LBB0_3:
+\tv_lshl_add_u64 v[4:5], v[2:3], 3, s[28:29]
+\tglobal_store_dwordx2 v[4:5], v[6:7], off
+\ts_endpgm
+.Lfunc_end0:
+"""
+GOOD = BAD.replace("""                                        ; implicit-def: $sgpr28_sgpr29
+\ts_nop 0""", "\ts_mov_b64 s[28:29], s[10:11]")
+
+
+def _check(text, tmp_path):
+    p = tmp_path / "k.s"
+    p.write_text(text)
+    return isa_check.check_file(str(p))
+
+
+def test_flags_store_base_undefined_on_one_path(tmp_path):
+    found = _check(BAD, tmp_path)
+    assert list(found) == ["_Z4kernv"]
+    assert {r[3] for r in found["_Z4kernv"]} == {28, 29}
+
+
+def test_passes_when_every_path_defines_the_base(tmp_path):
+    assert _check(GOOD, tmp_path) == {}
+
+
+def test_built_library_assembly_is_clean():
+    """The assembly written by the last build (if any) has no findings."""
+    asm = os.path.join(HERE, "..", "ch_swe_amd", "_build", "swe_lib_gfx950.s")
+    if not os.path.exists(asm):
+        import pytest
+
+        pytest.skip("library not built in this tree")
+    assert isa_check.check_file(asm) == {}
