@@ -363,7 +363,6 @@ struct Engine : EngineBase {
     Coef<T> cf{};
     T *rec[2] = {nullptr, nullptr};
     int ntx = 0, nty = 0, ntiles = 0, nseg = 0;
-    int *tile_segs = nullptr, *seg_expect = nullptr, *seg_cnt = nullptr;
     int4 *seg_desc = nullptr;
 
     size_t elem() const override { return sizeof(T); }
@@ -482,15 +481,8 @@ struct Engine : EngineBase {
         nty = ft.nty;
         ntiles = ft.ntiles;
         nseg = ft.nseg;
-        const std::vector<int> &ts = ft.tile_segs, &expect = ft.expect;
         const std::vector<int4> &segs = ft.segs;
-        if ((rc = alloc((void **)&tile_segs, ts.size() * sizeof(int)))) return rc;
-        if ((rc = alloc((void **)&seg_expect, expect.size() * sizeof(int)))) return rc;
-        if ((rc = alloc((void **)&seg_cnt, expect.size() * sizeof(int)))) return rc;
         if ((rc = alloc((void **)&seg_desc, segs.size() * sizeof(int4)))) return rc;
-        HIP_TRY(hipMemcpyAsync(tile_segs, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice, pl->stream));
-        HIP_TRY(hipMemcpyAsync(seg_expect, expect.data(), expect.size() * sizeof(int), hipMemcpyHostToDevice,
-                               pl->stream));
         HIP_TRY(hipMemcpyAsync(seg_desc, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice, pl->stream));
         HIP_TRY(hipStreamSynchronize(pl->stream));
         return SWE_OK;
@@ -508,10 +500,7 @@ struct Engine : EngineBase {
         a.H0 = T(c.h_mean);
         a.ntx = ntx;
         a.nty = nty;
-        a.tile_segs = tile_segs;
-        a.seg_expect = seg_expect;
         a.seg_desc = seg_desc;
-        a.seg_cnt = seg_cnt;
         a.order = O;
         a.xcd = (pl->cfg.flags & SWE_FLAG_LINEAR_BLOCKS) ? 0 : 1;
         return a;
@@ -551,10 +540,13 @@ struct Engine : EngineBase {
                 a.a_dt = T(A[st]);
                 a.b_dt = T(B[st]);
                 a.rec_in = rec[st & 1];
-                a.rec_out = rec[(st + 1) & 1];
                 if (st == 0) k_stage<T, O, TY, 1><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
                 else if (st == 3) k_stage<T, O, TY, 3><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
                 else k_stage<T, O, TY, 2><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
+                // records of the stage output for the next stage (none after
+                // the last stage of the last step)
+                if (st < 3) launch_records<O>(nexts[st], rec[(st + 1) & 1]);
+                else if (it + 1 < nsteps) launch_records<O>(v0, rec[0]);
             }
     }
 
