@@ -52,7 +52,7 @@ class _CConfig(ctypes.Structure):
                 ("coriolis_const", ctypes.c_int), ("radius", ctypes.c_double),
                 ("grav", ctypes.c_double), ("h_mean", ctypes.c_double),
                 ("omega", ctypes.c_double), ("axis", ctypes.c_double * 3),
-                ("flags", ctypes.c_int)]
+                ("flags", ctypes.c_int), ("block_threads", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -120,8 +120,9 @@ class Config:
 
     def __init__(self, N, order=42, precision=FP64, radius=6.371e6, grav=9.81,
                  h_mean=1000.0, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
-                 coriolis_const=False, unfused=False, linear_blocks=False):
+                 coriolis_const=False, unfused=False, linear_blocks=False, block_threads=0):
         self.N, self.order, self.precision = int(N), int(order), int(precision)
+        self.block_threads = int(block_threads)
         self.unfused = bool(unfused)
         self.linear_blocks = bool(linear_blocks)
         self.radius, self.grav, self.h_mean, self.omega = radius, grav, h_mean, omega
@@ -135,6 +136,7 @@ class Config:
         c.radius, c.grav, c.h_mean, c.omega = self.radius, self.grav, self.h_mean, self.omega
         c.axis[:] = self.axis
         c.flags = (FLAG_UNFUSED if self.unfused else 0) | (FLAG_LINEAR_BLOCKS if self.linear_blocks else 0)
+        c.block_threads = self.block_threads
         return c
 
 
@@ -234,13 +236,14 @@ class Rk4:
 
 def create_linear_swm_operator(N, order=42, precision=FP64, grav=9.81, h_mean=1000.0,
                                radius=6.371e6, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
-                               coriolis_const=False, unfused=False, linear_blocks=False):
+                               coriolis_const=False, unfused=False, linear_blocks=False, block_threads=0):
     """Factory mirroring create_linear_swm_operator
     (operator_swm_factory_mod.f90 L586-685) for the Ch staggering.
     unfused=True selects the multi-kernel diagnostic path; linear_blocks=True
-    the plain block order of the fused stage kernel."""
+    the plain block order of the fused stage kernel; block_threads (256 or
+    512, 0 = default) its workgroup size."""
     return OperatorSwmLin(Config(N, order, precision, radius, grav, h_mean, omega, axis,
-                                 coriolis_const, unfused, linear_blocks))
+                                 coriolis_const, unfused, linear_blocks, block_threads))
 
 
 class Ensemble:

@@ -363,6 +363,7 @@ struct Engine : EngineBase {
     Coef<T> cf{};
     T *rec[2] = {nullptr, nullptr};
     int ntx = 0, nty = 0, ntiles = 0, nseg = 0;
+    int nthr = F_THREADS;  // workgroup size of the fused kernels
     int4 *seg_desc = nullptr;
 
     size_t elem() const override { return sizeof(T); }
@@ -444,6 +445,7 @@ struct Engine : EngineBase {
         if ((rc = upload_map(1, order == 21 ? 2 : 3, {0, 1}, &maps.ch, &maps.n_ch))) return rc;
         if ((rc = upload_map(2, 1, {0, 1}, &maps.xyv, &maps.n_xyv))) return rc;
         use_fused = !(p->cfg.flags & SWE_FLAG_UNFUSED);
+        nthr = p->cfg.block_threads ? p->cfg.block_threads : F_THREADS;
         if ((rc = init_fused(g))) return rc;
         HIP_TRY(hipStreamSynchronize(pl->stream));
         return SWE_OK;
@@ -509,7 +511,7 @@ struct Engine : EngineBase {
     template <int O>
     void launch_records(const T *y, T *r)
     {
-        k_records<T, O, TY><<<nseg, F_THREADS, 0, pl->stream>>>(d, cf, y, r, seg_desc);
+        k_records<T, O, TY><<<nseg, nthr, 0, pl->stream>>>(d, cf, y, r, seg_desc);
     }
 
     template <int O>
@@ -519,7 +521,7 @@ struct Engine : EngineBase {
         StageArgs<T> a = stage_args<O>(in);
         a.out = out;
         a.rec_in = rec[0];
-        k_stage<T, O, TY, 0><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
+        k_stage<T, O, TY, 0><<<ntiles, nthr, 0, pl->stream>>>(a);
     }
 
     template <int O>
@@ -540,9 +542,9 @@ struct Engine : EngineBase {
                 a.a_dt = T(A[st]);
                 a.b_dt = T(B[st]);
                 a.rec_in = rec[st & 1];
-                if (st == 0) k_stage<T, O, TY, 1><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
-                else if (st == 3) k_stage<T, O, TY, 3><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
-                else k_stage<T, O, TY, 2><<<ntiles, F_THREADS, 0, pl->stream>>>(a);
+                if (st == 0) k_stage<T, O, TY, 1><<<ntiles, nthr, 0, pl->stream>>>(a);
+                else if (st == 3) k_stage<T, O, TY, 3><<<ntiles, nthr, 0, pl->stream>>>(a);
+                else k_stage<T, O, TY, 2><<<ntiles, nthr, 0, pl->stream>>>(a);
                 // records of the stage output for the next stage (none after
                 // the last stage of the last step)
                 if (st < 3) launch_records<O>(nexts[st], rec[(st + 1) & 1]);
@@ -709,6 +711,8 @@ int swe_plan_create(const swe_config *cfg, swe_plan **out)
     if (cfg->N > 4096) return fail(SWE_EINVAL, "N above 4096 is not supported");
     if (cfg->precision != SWE_FP64 && cfg->precision != SWE_FP32)
         return fail(SWE_EINVAL, "precision must be SWE_FP64 or SWE_FP32");
+    if (cfg->block_threads != 0 && cfg->block_threads != 256 && cfg->block_threads != 512)
+        return fail(SWE_EINVAL, "block_threads must be 0, 256 or 512");
     if (!(cfg->radius > 0) || !std::isfinite(cfg->grav) || !std::isfinite(cfg->h_mean) ||
         !std::isfinite(cfg->omega))
         return fail(SWE_EINVAL, "non-finite or non-positive physical constant");

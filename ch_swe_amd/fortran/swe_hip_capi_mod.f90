@@ -21,6 +21,7 @@ module swe_hip_capi_mod
         real(c_double) :: omega
         real(c_double) :: axis(3)
         integer(c_int) :: flags
+        integer(c_int) :: block_threads
     end type swe_config_t
 
     public :: swe_plan_create, swe_plan_destroy, swe_state_create, swe_state_destroy, &

@@ -144,6 +144,7 @@ contains
         cfg%omega = omega
         cfg%axis = [0.0_8, 0.0_8, 1.0_8]
         cfg%flags = 0
+        cfg%block_threads = 0
         call check(swe_plan_create(cfg, domain%plan), "create_domain_hip")
         domain%N = N
         domain%order = order
