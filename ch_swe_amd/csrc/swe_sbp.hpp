@@ -37,12 +37,39 @@ SWE_HD Line<T> line(const T *p, long s) { return Line<T>{p, s}; }
 template <typename A>
 SWE_HD auto ld(const A &f, int, int k) -> decltype(f(k)) { return f(k); }
 
+// ---- interior rows (no closure, no SAT); the full stencils below use the
+//      same expressions for their interior rows
+template <int ORDER, typename T, typename A>
+SWE_HD T d_i2c_in(const A &f, int k)
+{
+    if (ORDER == 21) return f(k + 1) - f(k);
+    return (T(27) * (f(k + 1) - f(k)) - (f(k + 2) - f(k - 1))) * T(1.0 / 24);
+}
+template <int ORDER, typename T, typename A>
+SWE_HD T d_c2i_in(const A &f, int k)
+{
+    if (ORDER == 21) return f(k) - f(k - 1);
+    return (T(27) * (f(k) - f(k - 1)) - (f(k + 1) - f(k - 2))) * T(1.0 / 24);
+}
+template <int ORDER, typename T, typename A>
+SWE_HD T i_i2c_in(const A &f, int k)
+{
+    if (ORDER == 21) return T(0.5) * (f(k + 1) + f(k));
+    return (T(9) * (f(k + 1) + f(k)) - (f(k + 2) + f(k - 1))) * T(1.0 / 16);
+}
+template <int ORDER, typename T, typename A>
+SWE_HD T i_c2i_in(const A &f, int k)
+{
+    if (ORDER == 21) return T(0.5) * (f(k) + f(k - 1));
+    return (T(9) * (f(k) + f(k - 1)) - (f(k + 1) + f(k - 2))) * T(1.0 / 16);
+}
+
 // ---- interfaces -> centres derivative (+ SAT with duplicates f[0], f[n+2])
 template <int ORDER, typename T, typename A>
 SWE_HD T d_i2c_pt(const A &f, int s, int n, int k)
 {
     if (ORDER == 21) {
-        T v = ld(f, s, k + 1) - ld(f, s, k);
+        T v = d_i2c_in<21, T>([&](int kk) { return ld(f, s, kk); }, k);
         // 0.5*R/h with R = (3/2, -1/2), h = (1, 1)
         if (k == 1) v += T(0.75) * (ld(f, s, 1) - ld(f, s, 0));
         if (k == 2) v += T(-0.25) * (ld(f, s, 1) - ld(f, s, 0));
@@ -70,7 +97,7 @@ SWE_HD T d_i2c_pt(const A &f, int s, int n, int k)
             v += T(S[r]) * (ld(f, s, n + 2) - ld(f, s, n + 1));
             return v;
         }
-        return (T(27) * (ld(f, s, k + 1) - ld(f, s, k)) - (ld(f, s, k + 2) - ld(f, s, k - 1))) * T(1.0 / 24);
+        return d_i2c_in<42, T>([&](int kk) { return ld(f, s, kk); }, k);
     }
 }
 
@@ -84,7 +111,7 @@ SWE_HD T d_c2i_pt(const A &f, int s, int n, int k)
         T v;
         if (k == 1) v = ld(f, s, 2) - ld(f, s, 1);
         else if (k == nx) v = ld(f, s, nx - 1) - ld(f, s, nx - 2);
-        else v = ld(f, s, k) - ld(f, s, k - 1);
+        else v = d_c2i_in<21, T>([&](int kk) { return ld(f, s, kk); }, k);
         // 0.5/h11 = 1; R = (3/2, -1/2)
         if (k == 1)
             v += T(1.5) * (ld(f, s, 1) - ld(f, s, 0)) - T(0.5) * (ld(f, s, 2) - ld(f, s, -1));
@@ -117,7 +144,7 @@ SWE_HD T d_c2i_pt(const A &f, int s, int n, int k)
                              T(R[2]) * (ld(f, s, nx - 3) - ld(f, s, nx + 2)));
             return v;
         }
-        return (T(27) * (ld(f, s, k) - ld(f, s, k - 1)) - (ld(f, s, k + 1) - ld(f, s, k - 2))) * T(1.0 / 24);
+        return d_c2i_in<42, T>([&](int kk) { return ld(f, s, kk); }, k);
     }
 }
 
@@ -125,7 +152,7 @@ SWE_HD T d_c2i_pt(const A &f, int s, int n, int k)
 template <int ORDER, typename T, typename A>
 SWE_HD T i_i2c_pt(const A &f, int s, int n, int k)
 {
-    if (ORDER == 21) return T(0.5) * (ld(f, s, k + 1) + ld(f, s, k));
+    if (ORDER == 21) return i_i2c_in<21, T>([&](int kk) { return ld(f, s, kk); }, k);
     constexpr double c13 = SBP42::c13, c14 = SBP42::c14;
     constexpr double Q[3][5] = {
         {0.5 + c13 + 2 * c14, 0.5 - 2 * c13 - 3 * c14, c13, c14, 0},
@@ -143,7 +170,7 @@ SWE_HD T i_i2c_pt(const A &f, int s, int n, int k)
         return T(Q[r][0]) * ld(f, s, n + 1) + T(Q[r][1]) * ld(f, s, n) + T(Q[r][2]) * ld(f, s, n - 1) +
                T(Q[r][3]) * ld(f, s, n - 2) + T(Q[r][4]) * ld(f, s, n - 3);
     }
-    return (T(9) * (ld(f, s, k + 1) + ld(f, s, k)) - (ld(f, s, k + 2) + ld(f, s, k - 1))) * T(1.0 / 16);
+    return i_i2c_in<42, T>([&](int kk) { return ld(f, s, kk); }, k);
 }
 
 // ---- centres -> interfaces interpolation (no halo reads)
@@ -154,7 +181,7 @@ SWE_HD T i_c2i_pt(const A &f, int s, int n, int k)
     if (ORDER == 21) {
         if (k == 1) return ld(f, s, 1);
         if (k == nx) return ld(f, s, nx - 1);
-        return T(0.5) * (ld(f, s, k) + ld(f, s, k - 1));
+        return i_c2i_in<21, T>([&](int kk) { return ld(f, s, kk); }, k);
     }
     constexpr double c13 = SBP42::c13, c14 = SBP42::c14;
     constexpr double Q[4][5] = {
@@ -174,7 +201,7 @@ SWE_HD T i_c2i_pt(const A &f, int s, int n, int k)
         return T(Q[r][0]) * ld(f, s, nx - 1) + T(Q[r][1]) * ld(f, s, nx - 2) + T(Q[r][2]) * ld(f, s, nx - 3) +
                T(Q[r][3]) * ld(f, s, nx - 4) + T(Q[r][4]) * ld(f, s, nx - 5);
     }
-    return (T(9) * (ld(f, s, k) + ld(f, s, k - 1)) - (ld(f, s, k + 1) + ld(f, s, k - 2))) * T(1.0 / 16);
+    return i_c2i_in<42, T>([&](int kk) { return ld(f, s, kk); }, k);
 }
 
 }  // namespace swe

@@ -61,7 +61,7 @@ typedef struct {
     double omega;       /* rotation rate [1/s] */
     double axis[3];     /* rotation axis */
     int flags;          /* SWE_FLAG_* bits */
-    int block_threads;  /* workgroup size of the fused kernels: 0 (default 256), 256 or 512 */
+    int block_threads;  /* workgroup size of the fused kernels: 0 (default 512), 256 or 512 */
 } swe_config;
 
 const char *swe_last_error(void);

@@ -38,9 +38,10 @@ def _op(N, order, precision=cs.FP64, **kw):
 
 # sizes chosen to exercise the fused tiling: single-tile panels (N=8), last
 # tile one point wide in y (FP64 tiles are 32x8, FP32 32x16: N=16, 32) and
-# in x (N=32), exact multiples (N=15, 31, 47), and larger panels
-APPLY_CASES = [(21, 8), (21, 16), (21, 32), (21, 33), (42, 8), (42, 15), (42, 16),
-               (42, 31), (42, 40), (42, 47)]
+# in x (N=32), exact multiples (N=15, 31, 47), and panels large enough for
+# tiles of the interior fast path (N=80, 97)
+APPLY_CASES = [(21, 8), (21, 16), (21, 32), (21, 33), (21, 80), (42, 8), (42, 15), (42, 16),
+               (42, 31), (42, 40), (42, 47), (42, 97)]
 
 
 @pytest.mark.parametrize("order,N", APPLY_CASES)
@@ -57,6 +58,22 @@ def test_apply_fp64_matches_oracle(oracle_mod, order, N, cor_const):
     op.apply(vout, vin)
     got = vout.download()
     for g, r in zip(got, ref):
+        assert _rel(g, r) < 1e-12
+
+
+@pytest.mark.parametrize("block_threads,linear", [(256, False), (512, True)])
+def test_launch_variants_match_oracle(oracle_mod, block_threads, linear):
+    """Workgroup size 256 (every tile on the general path) and the plain
+    block order give the same operator."""
+    N, order = 80, 42
+    _say(f"apply fp64 N={N} block_threads={block_threads} linear={linear}")
+    o = oracle_mod.Oracle(N, order)
+    u, v, h = _random_state(o, 5)
+    op = _op(N, order, block_threads=block_threads, linear_blocks=linear)
+    vin, vout = op.create_stvec(), op.create_stvec()
+    vin.upload(u, v, h)
+    op.apply(vout, vin)
+    for g, r in zip(vout.download(), o.apply(u, v, h)):
         assert _rel(g, r) < 1e-12
 
 
@@ -97,7 +114,7 @@ def test_rk4_config1_fp64_matches_oracle(oracle_mod, seed):
         assert _rel(g, r) < 1e-12
 
 
-@pytest.mark.parametrize("order,N", [(21, 24), (42, 24), (42, 40)])
+@pytest.mark.parametrize("order,N", [(21, 24), (42, 24), (42, 40), (42, 80)])
 def test_rk4_fp64_random_start_matches_oracle(oracle_mod, order, N):
     _say(f"rk4 random start order={order} N={N}")
     o = oracle_mod.Oracle(N, order)
@@ -111,12 +128,11 @@ def test_rk4_fp64_random_start_matches_oracle(oracle_mod, order, N):
         assert _rel(g, r) < 1e-12
 
 
-@pytest.mark.parametrize("order", [21, 42])
-def test_apply_fp32_close_to_oracle(oracle_mod, order):
+@pytest.mark.parametrize("order,N", [(21, 32), (42, 32), (42, 80)])
+def test_apply_fp32_close_to_oracle(oracle_mod, order, N):
     """FP32 stencils with FP64-built coefficients: one apply on smooth input
     agrees with FP64 to 1e-5 relative (max error over max magnitude; dh
     involves cancellation in the divergence)."""
-    N = 32
     _say(f"apply fp32 order={order}")
     o = oracle_mod.Oracle(N, order)
     h = sum(o.gaussian_hill(lam, phi, 0.4, 10.0) for lam, phi in CENTRES)
