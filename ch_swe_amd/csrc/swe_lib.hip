@@ -362,7 +362,7 @@ struct Engine : EngineBase {
     bool use_fused = true;
     Coef<T> cf{};
     T *rec[2] = {nullptr, nullptr};
-    int ntx = 0, nty = 0, ntiles = 0, nseg = 0;
+    int ntx = 0, nty = 0, ntiles = 0, nseg = 0, fwx = 0, fwy = 0;
     int nthr = F_THREADS;  // workgroup size of the fused kernels
     int4 *seg_desc = nullptr;
 
@@ -481,6 +481,8 @@ struct Engine : EngineBase {
         if (build_fused_tables(N, TY, ft, err)) return fail(SWE_EINVAL, err);
         ntx = ft.ntx;
         nty = ft.nty;
+        fwx = ft.fwx;
+        fwy = ft.fwy;
         ntiles = ft.ntiles;
         nseg = ft.nseg;
         const std::vector<int4> &segs = ft.segs;
@@ -502,6 +504,8 @@ struct Engine : EngineBase {
         a.H0 = T(c.h_mean);
         a.ntx = ntx;
         a.nty = nty;
+        a.fwx = fwx;
+        a.fwy = fwy;
         a.seg_desc = seg_desc;
         a.order = O;
         a.xcd = (pl->cfg.flags & SWE_FLAG_LINEAR_BLOCKS) ? 0 : 1;

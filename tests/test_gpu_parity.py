@@ -39,9 +39,9 @@ def _op(N, order, precision=cs.FP64, **kw):
 # sizes chosen to exercise the fused tiling: single-tile panels (N=8), last
 # tile one point wide in y (FP64 tiles are 32x8, FP32 32x16: N=16, 32) and
 # in x (N=32), exact multiples (N=15, 31, 47), and panels large enough for
-# tiles of the interior fast path (N=80, 97)
-APPLY_CASES = [(21, 8), (21, 16), (21, 32), (21, 33), (21, 80), (42, 8), (42, 15), (42, 16),
-               (42, 31), (42, 40), (42, 47), (42, 97)]
+# edge-aligned tiling with tiles on the interior fast path (N=48, 57, 80, 97)
+APPLY_CASES = [(21, 8), (21, 16), (21, 32), (21, 33), (21, 48), (21, 80), (42, 8), (42, 15),
+               (42, 16), (42, 31), (42, 40), (42, 47), (42, 57), (42, 97)]
 
 
 @pytest.mark.parametrize("order,N", APPLY_CASES)
