@@ -162,13 +162,14 @@ def main():
     cells = 6 * N * N
     ms_step = r32["ms"] / args.steps
     value = cells * args.steps / (r32["ms"] * 1e-3) / 1e6
-    # Dominant kernel: the fused stage kernel, 4 launches per step.  Its
-    # algorithmic traffic is 3, 5, 5, 3 state vectors for the four stages
-    # (stage 1 reads y writes acc, y2; stages 2-3 read y, v0, acc and write
-    # acc, y'; stage 4 reads y, acc and writes v0): 4 vectors per launch on
-    # average, one vector = DOF * 4 bytes.  The timed region holds the 4*K
-    # stage launches plus one edge-record launch for the initial state, whose
-    # duration (a few us) is charged to the stage launches.
+    # Dominant kernel: the fused stage kernel, 4 launches per step.  The
+    # algorithmic traffic of classic RK4 done stage by stage is 3, 5, 5, 3
+    # state vectors (stage 1 reads y writes acc, y2; stages 2-3 read y, v0,
+    # acc and write acc, y'; stage 4 reads y, acc and writes v0): 4 vectors
+    # per launch on average, one vector = DOF * 4 bytes.  (The kernels move
+    # 17 vectors a step: the last stage also reads v0, because acc holds only
+    # the increment.)  The timed region also holds the edge-record launches
+    # (one after each stage), which are charged to the stage launches.
     launch_bytes = 4 * dof(N) * 4
     launch_us = r32["ms"] * 1e3 / (4 * args.steps)
     achieved = launch_bytes / (launch_us * 1e-6) / 1e9
