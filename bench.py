@@ -81,7 +81,15 @@ def run_gpu(cs, precision, N, state0, dt, steps, warmup, ev):
     wall = time.perf_counter() - t0
     ms = ev.elapsed_ms(a, b)
     m1, e1 = op.integrals(st)
-    return dict(ms=ms, wall=wall, state=st.download(), energy_drift=e1 / e0 - 1.0)
+    out = dict(ms=ms, wall=wall, state=st.download(), energy_drift=e1 / e0 - 1.0)
+    # per-kernel durations from a separate pass with every stage / records
+    # launch bracketed by HIP events on the plan's stream
+    scratch = op.create_stvec().upload(*out["state"])
+    op.kernel_timing(True)
+    rk4.step(scratch, op, dt, min(steps, 200))
+    out["stage_ms"], out["n_stage"], out["records_ms"], out["n_records"] = op.kernel_times()
+    op.kernel_timing(False)
+    return out
 
 
 def rel_l2(a, b, w):
@@ -169,9 +177,10 @@ def main():
     # per launch on average, one vector = DOF * 4 bytes.  (The kernels move
     # 17 vectors a step: the last stage also reads v0, because acc holds only
     # the increment.)  The timed region also holds the edge-record launches
-    # (one after each stage), which are charged to the stage launches.
+    # (one after each stage).  Durations come from the event-bracketed pass.
     launch_bytes = 4 * dof(N) * 4
-    launch_us = r32["ms"] * 1e3 / (4 * args.steps)
+    launch_us = r32["stage_ms"] * 1e3 / r32["n_stage"]
+    records_us = r32["records_ms"] * 1e3 / max(r32["n_records"], 1)
     achieved = launch_bytes / (launch_us * 1e-6) / 1e9
 
     errs = None
@@ -211,6 +220,8 @@ def main():
             "kernel": "k_stage (fused RK4 stage), mean of the 4 stage launches per step",
             "bytes_per_launch": launch_bytes,
             "us_per_launch": round(launch_us, 3),
+            "records_us_per_launch": round(records_us, 3),
+            "step_fraction_in_stage_kernel": round(4 * launch_us * 1e-3 / ms_step, 3),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

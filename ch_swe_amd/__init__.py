@@ -40,6 +40,7 @@ EXPORTS = (
     "swe_state_upload", "swe_state_download", "swe_apply", "swe_assign", "swe_update",
     "swe_rk4_steps", "swe_integrals", "swe_halo_map", "swe_ensemble_create",
     "swe_ensemble_destroy", "swe_ensemble_rk4", "swe_ensemble_stream", "swe_ensemble_synchronize",
+    "swe_plan_timing", "swe_plan_kernel_times",
 )
 
 
@@ -94,6 +95,8 @@ def lib():
     L.swe_ensemble_stream.argtypes = [_vp]
     L.swe_ensemble_stream.restype = _vp
     L.swe_ensemble_synchronize.argtypes = [_vp]
+    L.swe_plan_timing.argtypes = [_vp, ctypes.c_int]
+    L.swe_plan_kernel_times.argtypes = [_vp, _dp, _ip, _dp, _ip]
     _lib = L
     return L
 
@@ -225,6 +228,18 @@ class OperatorSwmLin:
     @property
     def stream(self):
         return lib().swe_plan_stream(self.plan.h)
+
+    def kernel_timing(self, enable=True):
+        """Bracket every fused stage / records launch with HIP events."""
+        _check(lib().swe_plan_timing(self.plan.h, 1 if enable else 0))
+
+    def kernel_times(self):
+        """(stage_ms, n_stage, records_ms, n_records) since the last call."""
+        sm, rm = ctypes.c_double(), ctypes.c_double()
+        sn, rn = ctypes.c_int(), ctypes.c_int()
+        _check(lib().swe_plan_kernel_times(self.plan.h, ctypes.byref(sm), ctypes.byref(sn),
+                                           ctypes.byref(rm), ctypes.byref(rn)))
+        return sm.value, sn.value, rm.value, rn.value
 
 
 class Rk4:

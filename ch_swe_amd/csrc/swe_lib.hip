@@ -311,7 +311,70 @@ struct DevBuf {
     }
 };
 
+// Optional per-launch timing of the fused kernels (measurement aid): HIP
+// events recorded on the plan stream around every stage and record launch.
+struct KernelTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    struct Span {
+        hipEvent_t a, b;
+        int kind;  // 0 stage kernel, 1 records kernel
+    };
+    std::vector<Span> spans;
+    ~KernelTimer()
+    {
+        for (auto &s : spans) {
+            (void)hipEventDestroy(s.a);
+            (void)hipEventDestroy(s.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t get()
+    {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    // bracket one launch: begin() before, end() after
+    hipEvent_t begin(hipStream_t s)
+    {
+        if (!on) return nullptr;
+        hipEvent_t a = get();
+        (void)hipEventRecord(a, s);
+        return a;
+    }
+    void end(hipEvent_t a, hipStream_t s, int kind)
+    {
+        if (!on || !a) return;
+        hipEvent_t b = get();
+        (void)hipEventRecord(b, s);
+        spans.push_back(Span{a, b, kind});
+    }
+    int collect(double *ms, int *cnt)
+    {
+        ms[0] = ms[1] = 0;
+        cnt[0] = cnt[1] = 0;
+        for (auto &sp : spans) {
+            HIP_TRY(hipEventSynchronize(sp.b));
+            float t = 0;
+            HIP_TRY(hipEventElapsedTime(&t, sp.a, sp.b));
+            ms[sp.kind] += t;
+            cnt[sp.kind]++;
+            pool.push_back(sp.a);
+            pool.push_back(sp.b);
+        }
+        spans.clear();
+        return SWE_OK;
+    }
+};
+
 struct EngineBase {
+    KernelTimer timer;
     virtual ~EngineBase() = default;
     virtual int apply(void *out, void *in) = 0;
     virtual int axpy(void *out, double s1, void *x, double s2, void *y, int update) = 0;
@@ -515,7 +578,9 @@ struct Engine : EngineBase {
     template <int O>
     void launch_records(const T *y, T *r)
     {
+        hipEvent_t e = timer.begin(pl->stream);
         k_records<T, O, TY><<<nseg, nthr, 0, pl->stream>>>(d, cf, y, r, seg_desc);
+        timer.end(e, pl->stream, 1);
     }
 
     template <int O>
@@ -525,7 +590,9 @@ struct Engine : EngineBase {
         StageArgs<T> a = stage_args<O>(in);
         a.out = out;
         a.rec_in = rec[0];
+        hipEvent_t e = timer.begin(pl->stream);
         k_stage<T, O, TY, 0><<<ntiles, nthr, 0, pl->stream>>>(a);
+        timer.end(e, pl->stream, 0);
     }
 
     template <int O>
@@ -546,9 +613,11 @@ struct Engine : EngineBase {
                 a.a_dt = T(A[st]);
                 a.b_dt = T(B[st]);
                 a.rec_in = rec[st & 1];
+                hipEvent_t e = timer.begin(pl->stream);
                 if (st == 0) k_stage<T, O, TY, 1><<<ntiles, nthr, 0, pl->stream>>>(a);
                 else if (st == 3) k_stage<T, O, TY, 3><<<ntiles, nthr, 0, pl->stream>>>(a);
                 else k_stage<T, O, TY, 2><<<ntiles, nthr, 0, pl->stream>>>(a);
+                timer.end(e, pl->stream, 0);
                 // records of the stage output for the next stage (none after
                 // the last stage of the last step)
                 if (st < 3) launch_records<O>(nexts[st], rec[(st + 1) & 1]);
@@ -867,6 +936,30 @@ int swe_rk4_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps)
         if ((rc = swe_update(plan, v0, dt / 6.0, k1, dt / 3.0, k2))) return rc;
         if ((rc = swe_update(plan, v0, dt / 3.0, k3, dt / 6.0, k4))) return rc;
     }
+    return SWE_OK;
+}
+
+int swe_plan_timing(swe_plan *plan, int enable)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    double ms[2];
+    int cnt[2];
+    int rc = plan->eng->timer.collect(ms, cnt);  // drop what was recorded before
+    plan->eng->timer.on = enable != 0;
+    return rc;
+}
+
+int swe_plan_kernel_times(swe_plan *plan, double *stage_ms, int *n_stage, double *records_ms, int *n_records)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    double ms[2];
+    int cnt[2];
+    int rc = plan->eng->timer.collect(ms, cnt);
+    if (rc) return rc;
+    if (stage_ms) *stage_ms = ms[0];
+    if (n_stage) *n_stage = cnt[0];
+    if (records_ms) *records_ms = ms[1];
+    if (n_records) *n_records = cnt[1];
     return SWE_OK;
 }
 

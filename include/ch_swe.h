@@ -106,6 +106,15 @@ int swe_rk4_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps);
  * energy = g/2 <h,h> + H0/2 (<u,ut> + <v,vt>). */
 int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy);
 
+/* Measurement aid (no counterpart in the reference): with timing enabled,
+ * every launch of the fused stage kernel and of the edge-record kernel on
+ * this plan is bracketed by HIP events on the plan's stream.
+ * swe_plan_kernel_times waits for them, returns the summed durations (ms)
+ * and counts since the last call, and starts a new interval.  Enabling or
+ * disabling drops what was recorded. */
+int swe_plan_timing(swe_plan *plan, int enable);
+int swe_plan_kernel_times(swe_plan *plan, double *stage_ms, int *n_stage, double *records_ms, int *n_records);
+
 /* Host-only (no device needed): halo map of the in-device edge gathers, for
  * inspection and tests.  kind 0: xy scalar depth 1; 1: Ch vector (u across
  * left/right, v across bottom/top) depth 2 (sbp21) or 3 (sbp42); 2: xy
