@@ -219,3 +219,46 @@ def test_full_size_invariants(oracle_mod, precision):
              + o.h_mean * np.sqrt((np.sum(ut * ut * wu) + np.sum(vt * vt * wv))
                                   * (np.sum(du * du * wu) + np.sum(dv * dv * wv))))
     assert abs(dE / scale) < (1e-12 if precision == cs.FP64 else 1e-5)
+
+
+def _continuous_h(N, order, h):
+    """Make the duplicated edge values of h bitwise equal (copy from the
+    lowest-numbered panel), as the reference's exchanged halos hold them."""
+    m = cs.halo_map(N, order, 0)
+    h = h.copy()
+    for _ in range(3):
+        for tp, ti, tj, sp, si, sj in zip(m["tp"], m["ti"], m["tj"], m["sp"], m["si"], m["sj"]):
+            ti = 1 if ti == 0 else (N + 1 if ti == N + 2 else ti)
+            tj = 1 if tj == 0 else (N + 1 if tj == N + 2 else tj)
+            if sp < tp:
+                h[tp, tj - 1, ti - 1] = h[sp, sj - 1, si - 1]
+            else:
+                h[sp, sj - 1, si - 1] = h[tp, tj - 1, ti - 1]
+    return h
+
+
+def _dup_mismatch(N, order, h):
+    m = cs.halo_map(N, order, 0)
+    worst = 0.0
+    for tp, ti, tj, sp, si, sj in zip(m["tp"], m["ti"], m["tj"], m["sp"], m["si"], m["sj"]):
+        ti = 1 if ti == 0 else (N + 1 if ti == N + 2 else ti)
+        tj = 1 if tj == 0 else (N + 1 if tj == N + 2 else tj)
+        worst = max(worst, abs(h[tp, tj - 1, ti - 1] - h[sp, sj - 1, si - 1]))
+    return worst
+
+
+@pytest.mark.parametrize("precision", [cs.FP64, cs.FP32])
+@pytest.mark.parametrize("order,N", [(21, 33), (42, 48)])
+def test_duplicated_h_stays_bitwise_continuous(oracle_mod, precision, order, N):
+    """Every panel sharing an edge point computes the same projected h
+    tendency from the same numbers, so duplicated h values stay identical
+    (bitwise) through RK4 steps, in FP64 and FP32."""
+    _say(f"continuity precision={precision} order={order} N={N}")
+    o = oracle_mod.Oracle(N, order)
+    u, v, _ = _random_state(o, 9)
+    h = _continuous_h(N, order, o.gaussian_hill(0.8, 0.3, 0.4, 10.0))
+    assert _dup_mismatch(N, order, h) == 0.0
+    op = _op(N, order, precision)
+    st = op.create_stvec().upload(u * 1e-3, v * 1e-3, h)
+    cs.create_timescheme("rk4").step(st, op, 600.0, 20)
+    assert _dup_mismatch(N, order, st.download()[2]) == 0.0
