@@ -117,6 +117,112 @@ def host_threads():
     return len(os.sched_getaffinity(0))
 
 
+def bench_ensemble(args, ev):
+    """Uniform batched ensemble: 256 members at N=96, sbp42, FP32 state, all
+    members' stages in one launch each, the step loop replayed from captured
+    HIP graphs; 500 timed steps."""
+    from ch_swe_amd import testcases as tc
+    import ch_swe_amd as cs
+
+    N, members, steps = 96, args.members, args.steps
+    states = tc.ensemble_states(N, ORDER, members, lmax=16, seed=1)
+    dt = tc.cfl_dt(N, 0.5)
+    op = cs.create_linear_swm_operator(N, ORDER, cs.FP32)
+    mem = [op.create_stvec().upload(*s) for s in states]
+    ens = cs.Ensemble(mem)
+    ens.rk4(dt, args.warmup, graph=True)
+    ens.synchronize()
+    a, b = ev.create(), ev.create()
+    t0 = time.perf_counter()
+    ev.record(a, ens.stream)
+    ens.rk4(dt, steps, graph=True)
+    ev.record(b, ens.stream)
+    ens.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev.elapsed_ms(a, b)
+    cells = 6 * N * N
+    value = members * cells * steps / (ms * 1e-3) / 1e6
+    # one member alone, same steps, for the per-member agreement check
+    solo = op.create_stvec().upload(*states[0])
+    cs.create_timescheme("rk4").step(solo, op, dt, args.warmup + steps)
+    d = max(float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-30)) for x, y in zip(mem[0].download(), solo.download()))
+    return {
+        "metric": "aggregate Mpts*steps/s, uniform batched ensemble (FP32 stencils)",
+        "value": round(value, 2), "unit": "Mpts*steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(ms / steps, 5), "wall_ms_per_step": round(wall * 1e3 / steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded degree-16 spherical-harmonic fields + Gaussian bumps, one per member)",
+        "config": {"workload": f"uniform ensemble: {members} members, N={N}, sbp42, RK4 at CFL 0.5, HIP-graph step loop",
+                   "members": members, "N": N, "order": ORDER, "dt_s": round(dt, 4)},
+        "member0_vs_alone_max_rel_diff": d,
+    }
+
+
+def bench_uneven(args, ev):
+    """Uneven batch: 64 instances, N in {24,48,96,192}, sbp21/sbp42 mixed,
+    own CFL dt, steps log-uniform in [100, 2000], FP32; batched run against
+    the same instances solved one at a time."""
+    from ch_swe_amd import testcases as tc
+    import ch_swe_amd as cs
+
+    rng = np.random.default_rng(4)
+    n_inst = args.members
+    Ns = rng.choice([24, 48, 96, 192], n_inst)
+    orders = rng.choice([21, 42], n_inst)
+    steps = np.round(np.exp(rng.uniform(np.log(100), np.log(2000), n_inst))).astype(int)
+    if args.steps_scale != 1.0:
+        steps = np.maximum(1, np.round(steps * args.steps_scale)).astype(int)
+    dts = np.array([tc.cfl_dt(int(N), 0.5) for N in Ns])
+    plans, init = {}, []
+    for i in range(n_inst):
+        key = (int(Ns[i]), int(orders[i]))
+        if key not in plans:
+            plans[key] = cs.create_linear_swm_operator(key[0], key[1], cs.FP32)
+        init.append(tc.ensemble_states(key[0], key[1], 1, lmax=16, seed=100 + i, basis=1)[0])
+    cells = np.array([6 * int(N) * int(N) for N in Ns])
+    work = float(np.sum(cells * steps))
+    # batched
+    mem = [plans[(int(Ns[i]), int(orders[i]))].create_stvec().upload(*init[i]) for i in range(n_inst)]
+    ens = cs.Ensemble(mem)
+    ens.rk4(dts, np.minimum(steps, 5), graph=False)  # warm-up (compiles nothing; touches memory)
+    ens.synchronize()
+    for i in range(n_inst):
+        mem[i].upload(*init[i])
+    a, b = ev.create(), ev.create()
+    ev.record(a, ens.stream)
+    ens.rk4(dts, steps, graph=True)
+    ev.record(b, ens.stream)
+    ens.synchronize()
+    ms_batch = ev.elapsed_ms(a, b)
+    # one at a time
+    solo = [plans[(int(Ns[i]), int(orders[i]))].create_stvec().upload(*init[i]) for i in range(n_inst)]
+    rk = cs.create_timescheme("rk4")
+    for p in plans.values():
+        p.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_inst):
+        op = plans[(int(Ns[i]), int(orders[i]))]
+        rk.step(solo[i], op, float(dts[i]), int(steps[i]))
+        op.synchronize()
+    ms_seq = (time.perf_counter() - t0) * 1e3
+    d = max(float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-30))
+            for i in range(n_inst) for x, y in zip(mem[i].download(), solo[i].download()))
+    return {
+        "metric": "aggregate Mpts*steps/s, uneven batch (FP32 stencils)",
+        "value": round(work / (ms_batch * 1e-3) / 1e6, 2), "unit": "Mpts*steps/s", "n_gpus": 1,
+        "steps": int(steps.max()), "warmup": 5, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded degree-16 spherical-harmonic field + Gaussian bump per instance)",
+        "config": {"workload": f"uneven batch: {n_inst} instances, N in {{24,48,96,192}}, sbp21/42, own dt, "
+                               "steps log-uniform [100, 2000], HIP-graph step loop",
+                   "instances": n_inst, "cells_x_steps": work},
+        "batched_ms": round(ms_batch, 3), "one_at_a_time_ms": round(ms_seq, 3),
+        "one_at_a_time_value": round(work / (ms_seq * 1e-3) / 1e6, 2),
+        "speedup_vs_one_at_a_time": round(ms_seq / ms_batch, 2),
+        "batched_vs_alone_max_rel_diff": d,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +232,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=host_threads())
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--workload", choices=["single", "ensemble", "uneven"], default="single",
+                    help="single large solve (default, the headline), uniform ensemble, uneven batch")
+    ap.add_argument("--members", type=int, default=None, help="ensemble members / batch instances")
+    ap.add_argument("--steps-scale", type=float, default=1.0, help="uneven batch: scale every step count")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (3 launches) so that a "
                          "counter profile can scale FETCH_SIZE / WRITE_SIZE by a known size")
@@ -134,6 +244,19 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "single":
+        ev = HipEvents()
+        if args.workload == "ensemble":
+            args.members = args.members or 256
+            if args.steps == 1000:
+                args.steps = 500
+            line = bench_ensemble(args, ev)
+        else:
+            args.members = args.members or 64
+            line = bench_uneven(args, ev)
+        print(json.dumps(line), flush=True)
+        return
 
     from ch_swe_amd import testcases as tc
 

@@ -196,6 +196,32 @@ def sh_state(N, order=42, lmax=32, seed=0, rms=20.0, grav=9.81, omega=7.292e-5, 
     return out["u"], out["v"], out["h"]
 
 
+def ensemble_states(N, order, members, lmax=16, seed=0, basis=8, rms=20.0, radius=6.371e6):
+    """Seeded ensemble initial states: member m gets a degree-lmax
+    spherical-harmonic field with coefficients ~ N(0,1) l^-2 (a unit-norm
+    random combination of `basis` sh_state fields, which has the same
+    coefficient distribution) with its balanced winds, plus a Gaussian bump of
+    amplitude U(1, 20) m, width U(2e5, 1e6) m and a centre uniform on the
+    sphere (no wind).  Returns a list of (u, v, h)."""
+    base = [sh_state(N, order, lmax=lmax, seed=seed * 1000 + k, rms=rms) for k in range(basis)]
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((members, basis))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    amp = rng.uniform(1.0, 20.0, members)
+    width = rng.uniform(2e5, 1e6, members) / radius
+    cen = rng.standard_normal((members, 3))
+    cen /= np.linalg.norm(cen, axis=1, keepdims=True)
+    r, _, _ = mesh_geometry(N, "h")
+    out = []
+    for m in range(members):
+        u = sum(w[m, k] * base[k][0] for k in range(basis))
+        v = sum(w[m, k] * base[k][1] for k in range(basis))
+        h = sum(w[m, k] * base[k][2] for k in range(basis))
+        d = np.arccos(np.clip(r @ cen[m], -1.0, 1.0))
+        out.append((u, v, h + amp[m] * np.exp(-((d / width[m]) ** 2))))
+    return out
+
+
 def cfl_dt(N, courant=0.5, grav=9.81, h_mean=1000.0, radius=6.371e6):
     """dt with c dt / (a hx) = courant, c = sqrt(g H0) (the CFL definition of
     swm_output_diag_mod.f90 L138-149 with h -> H0)."""
