@@ -223,6 +223,81 @@ def bench_uneven(args, ev):
     }
 
 
+def bench_serving(args, ev):
+    """Serving: Poisson request streams (rates swept) of solves with N in
+    {32,64,128,256,512} (weights 0.3/0.3/0.2/0.15/0.05), steps log-uniform in
+    [100, 2000], FP32 or FP64 per request, seeded h0 (scaled degree-16
+    spherical-harmonic field with balanced winds + Gaussian bump); admitted
+    by a device-memory budget, continuously batched.  Reports p50/p99
+    latency (arrival to result) and aggregate Mpts*steps/s per rate."""
+    from ch_swe_amd import testcases as tc
+    import ch_swe_amd as cs
+
+    sizes, weights = [32, 64, 128, 256, 512], [0.3, 0.3, 0.2, 0.15, 0.05]
+    base = {N: tc.sh_state(N, ORDER, lmax=16, seed=7, rms=20.0, workers=1) for N in sizes}
+    geo = {N: tc.mesh_geometry(N, "h")[0] for N in sizes}
+    rates = [float(r) for r in args.rates.split(",")]
+    n_req = args.members or 48
+    rows = []
+    for ri, lam in enumerate(rates):
+        # the request stream, built before the clock starts (client side)
+        rng = np.random.default_rng(100 + ri)
+        arr = np.cumsum(rng.exponential(1.0 / lam, n_req))
+        Ns = rng.choice(sizes, n_req, p=weights)
+        steps = np.round(np.exp(rng.uniform(np.log(100), np.log(2000), n_req))).astype(int)
+        precs = rng.choice([cs.FP32, cs.FP64], n_req)
+        scale = rng.standard_normal(n_req)
+        amp, width = rng.uniform(1, 20, n_req), rng.uniform(2e5, 1e6, n_req) / 6.371e6
+        cen = rng.standard_normal((n_req, 3))
+        cen /= np.linalg.norm(cen, axis=1, keepdims=True)
+        states = []
+        for k in range(n_req):
+            u, v, h = base[int(Ns[k])]
+            d = np.arccos(np.clip(geo[int(Ns[k])] @ cen[k], -1.0, 1.0))
+            states.append((scale[k] * u, scale[k] * v, scale[k] * h + amp[k] * np.exp(-((d / width[k]) ** 2))))
+        srv = cs.Server(int(args.budget_gb * (1 << 30)))
+        ids, t_arr, t_done = {}, {}, {}
+        nxt, t0 = 0, time.perf_counter()
+        while len(t_done) < n_req:
+            now = time.perf_counter() - t0
+            while nxt < n_req and arr[nxt] <= now:
+                N = int(Ns[nxt])
+                rid = srv.submit(*states[nxt], tc.cfl_dt(N, 0.5), int(steps[nxt]), N, ORDER, int(precs[nxt]))
+                ids[rid] = nxt
+                t_arr[rid] = arr[nxt]
+                nxt += 1
+            q, a, _ = srv.stats()
+            if q == 0 and a == 0:
+                if nxt < n_req:
+                    time.sleep(max(0.0, arr[nxt] - (time.perf_counter() - t0)))
+                continue
+            srv.tick(args.chunk)
+            tnow = time.perf_counter() - t0
+            for rid in [r for r in ids if r not in t_done]:
+                if srv.take(rid) is not None:
+                    t_done[rid] = tnow
+        makespan = max(t_done.values()) - arr[0]
+        lat = np.array([t_done[r] - t_arr[r] for r in t_done]) * 1e3
+        work = float(np.sum([6 * int(N) * int(N) * int(s) for N, s in zip(Ns, steps)]))
+        rows.append({"rate_per_s": lam, "requests": n_req, "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                     "mpts_steps_per_s": round(work / makespan / 1e6, 2),
+                     "fp64_share": round(float(np.mean(precs == cs.FP64)), 2)})
+        print(f"# serving rate {lam}/s: {rows[-1]}", file=sys.stderr, flush=True)
+        srv.close()
+    top = rows[-1]
+    return {
+        "metric": "p50/p99 latency (ms) and aggregate Mpts*steps/s, served mixed-size solves",
+        "value": top["mpts_steps_per_s"], "unit": "Mpts*steps/s", "n_gpus": 1, "steps": None,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic Poisson request stream (seeded sizes, steps, precisions, initial states)",
+        "config": {"workload": f"serving: Poisson arrivals at {args.rates} req/s, {n_req} requests per rate, "
+                               f"budget {args.budget_gb} GB, {args.chunk}-step ticks",
+                   "sizes": sizes, "weights": weights},
+        "per_rate": rows,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,8 +307,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=host_threads())
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", choices=["single", "ensemble", "uneven"], default="single",
-                    help="single large solve (default, the headline), uniform ensemble, uneven batch")
+    ap.add_argument("--workload", choices=["single", "ensemble", "uneven", "serving"], default="single",
+                    help="single large solve (default, the headline), uniform ensemble, uneven batch, serving")
+    ap.add_argument("--rates", default="5,10,20,40", help="serving: request rates (per second)")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serving: device-memory budget")
+    ap.add_argument("--chunk", type=int, default=25, help="serving: steps per tick")
     ap.add_argument("--members", type=int, default=None, help="ensemble members / batch instances")
     ap.add_argument("--steps-scale", type=float, default=1.0, help="uneven batch: scale every step count")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -252,6 +330,8 @@ def main():
             if args.steps == 1000:
                 args.steps = 500
             line = bench_ensemble(args, ev)
+        elif args.workload == "serving":
+            line = bench_serving(args, ev)
         else:
             args.members = args.members or 64
             line = bench_uneven(args, ev)

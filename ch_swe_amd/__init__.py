@@ -25,7 +25,7 @@ import numpy as np
 
 __all__ = [
     "SweError", "lib", "Config", "OperatorSwmLin", "StvecSwm", "Rk4",
-    "create_linear_swm_operator", "create_timescheme", "halo_map", "Ensemble",
+    "create_linear_swm_operator", "create_timescheme", "halo_map", "Ensemble", "Server",
     "FP64", "FP32", "LIB_PATH",
 ]
 
@@ -40,7 +40,8 @@ EXPORTS = (
     "swe_state_upload", "swe_state_download", "swe_apply", "swe_assign", "swe_update",
     "swe_rk4_steps", "swe_integrals", "swe_halo_map", "swe_ensemble_create",
     "swe_ensemble_destroy", "swe_ensemble_rk4", "swe_ensemble_stream", "swe_ensemble_synchronize",
-    "swe_plan_timing", "swe_plan_kernel_times",
+    "swe_plan_timing", "swe_plan_kernel_times", "swe_server_create", "swe_server_destroy",
+    "swe_server_submit", "swe_server_tick", "swe_server_take", "swe_server_stats",
 )
 
 
@@ -97,6 +98,13 @@ def lib():
     L.swe_ensemble_synchronize.argtypes = [_vp]
     L.swe_plan_timing.argtypes = [_vp, ctypes.c_int]
     L.swe_plan_kernel_times.argtypes = [_vp, _dp, _ip, _dp, _ip]
+    L.swe_server_create.argtypes = [ctypes.c_size_t, ctypes.POINTER(_vp)]
+    L.swe_server_destroy.argtypes = [_vp]
+    L.swe_server_submit.argtypes = [_vp, ctypes.POINTER(_CConfig), _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_longlong)]
+    L.swe_server_tick.argtypes = [_vp, ctypes.c_int, _ip]
+    L.swe_server_take.argtypes = [_vp, ctypes.c_longlong, _dp, _dp, _dp, _ip]
+    L.swe_server_stats.argtypes = [_vp, _ip, _ip, ctypes.POINTER(ctypes.c_size_t)]
     _lib = L
     return L
 
@@ -296,6 +304,58 @@ class Ensemble:
     @property
     def stream(self):
         return lib().swe_ensemble_stream(self.h)
+
+
+class Server:
+    """Continuous batching of independent solves under a device-memory
+    budget (swe_server_*)."""
+
+    def __init__(self, budget_bytes):
+        h = _vp()
+        _check(lib().swe_server_create(int(budget_bytes), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().swe_server_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, u, v, h, dt, nsteps, N, order=42, precision=FP64, **kw):
+        """Queue a solve; returns its id.  kw: Config keywords (physical
+        constants)."""
+        cfg = Config(N, order, precision, **kw)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, h)]
+        rid = ctypes.c_longlong()
+        _check(lib().swe_server_submit(self.h, ctypes.byref(cfg._c()), *[_p(a) for a in arrs], float(dt),
+                                       int(nsteps), ctypes.byref(rid)))
+        self._shapes = getattr(self, "_shapes", {})
+        self._shapes[rid.value] = N
+        return rid.value
+
+    def tick(self, chunk_steps=25):
+        """Admit, advance up to chunk_steps, retire; returns completions."""
+        n = ctypes.c_int()
+        _check(lib().swe_server_tick(self.h, int(chunk_steps), ctypes.byref(n)))
+        return n.value
+
+    def take(self, rid):
+        """(u, v, h) of a finished request, or None if it is still running."""
+        N = self._shapes[rid]
+        out = [np.empty((6, N + 1, N)), np.empty((6, N, N + 1)), np.empty((6, N + 1, N + 1))]
+        ready = ctypes.c_int()
+        _check(lib().swe_server_take(self.h, int(rid), *[_p(a) for a in out], ctypes.byref(ready)))
+        if not ready.value:
+            return None
+        del self._shapes[rid]
+        return tuple(out)
+
+    def stats(self):
+        q, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        _check(lib().swe_server_stats(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(b)))
+        return q.value, a.value, b.value
 
 
 def create_timescheme(name):

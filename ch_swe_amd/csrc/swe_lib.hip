@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <deque>
+#include <map>
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
@@ -992,3 +994,4 @@ int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *
 
 }  // extern "C"
 #include "swe_ensemble.inc"
+#include "swe_server.inc"

@@ -19,6 +19,8 @@
 #ifndef CH_SWE_H
 #define CH_SWE_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -105,6 +107,28 @@ int swe_rk4_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps);
  * L22-62, abstract_quadrature_mod.f90 L29-66): mass = sum h J hx hy Ax Ay;
  * energy = g/2 <h,h> + H0/2 (<u,ut> + <v,vt>). */
 int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy);
+
+/* Serving: continuous batching of independent solves of mixed size and
+ * precision under a device-memory budget (no counterpart in the reference,
+ * which runs one solve per program).  Requests are queued with a copy of
+ * their initial state; each tick admits queued requests in arrival order
+ * while their device memory fits the budget, advances every active request by
+ * up to chunk_steps RK4 steps (all active requests of one precision together,
+ * one launch per stage and SBP order, the two precisions on their own
+ * streams), and retires the finished ones.  Every request computes exactly
+ * what swe_rk4_steps computes for it alone.  Single-threaded use. */
+typedef struct swe_server swe_server;
+int swe_server_create(size_t budget_bytes, swe_server **out);
+int swe_server_destroy(swe_server *srv);
+/* cfg as for swe_plan_create (fused path); u, v, h packed FP64 (copied). */
+int swe_server_submit(swe_server *srv, const swe_config *cfg, const double *u, const double *v,
+                      const double *h, double dt, int nsteps, long long *id);
+/* Admit, advance, retire; *completed = requests finished in this call. */
+int swe_server_tick(swe_server *srv, int chunk_steps, int *completed);
+/* *ready = 1 and the final (u, v, h) copied out (and the request forgotten)
+ * once request id has finished; *ready = 0 before. */
+int swe_server_take(swe_server *srv, long long id, double *u, double *v, double *h, int *ready);
+int swe_server_stats(swe_server *srv, int *queued, int *active, size_t *bytes_in_use);
 
 /* Measurement aid (no counterpart in the reference): with timing enabled,
  * every launch of the fused stage kernel and of the edge-record kernel on
