@@ -120,7 +120,7 @@ def _p(a):
 
 
 # swe_config.flags and swe_ensemble_rk4 flags (include/ch_swe.h)
-FLAG_UNFUSED, FLAG_LINEAR_BLOCKS = 1, 2
+FLAG_UNFUSED, FLAG_LINEAR_BLOCKS, FLAG_GENERAL_TILES = 1, 2, 4
 ENS_GRAPH = 1
 
 
@@ -131,8 +131,10 @@ class Config:
 
     def __init__(self, N, order=42, precision=FP64, radius=6.371e6, grav=9.81,
                  h_mean=1000.0, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
-                 coriolis_const=False, unfused=False, linear_blocks=False, block_threads=0):
+                 coriolis_const=False, unfused=False, linear_blocks=False, block_threads=0,
+                 general_tiles=False):
         self.N, self.order, self.precision = int(N), int(order), int(precision)
+        self.general_tiles = bool(general_tiles)
         self.block_threads = int(block_threads)
         self.unfused = bool(unfused)
         self.linear_blocks = bool(linear_blocks)
@@ -146,7 +148,8 @@ class Config:
         c.coriolis_const = 1 if self.coriolis_const else 0
         c.radius, c.grav, c.h_mean, c.omega = self.radius, self.grav, self.h_mean, self.omega
         c.axis[:] = self.axis
-        c.flags = (FLAG_UNFUSED if self.unfused else 0) | (FLAG_LINEAR_BLOCKS if self.linear_blocks else 0)
+        c.flags = ((FLAG_UNFUSED if self.unfused else 0) | (FLAG_LINEAR_BLOCKS if self.linear_blocks else 0) |
+                   (FLAG_GENERAL_TILES if self.general_tiles else 0))
         c.block_threads = self.block_threads
         return c
 
@@ -259,14 +262,16 @@ class Rk4:
 
 def create_linear_swm_operator(N, order=42, precision=FP64, grav=9.81, h_mean=1000.0,
                                radius=6.371e6, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
-                               coriolis_const=False, unfused=False, linear_blocks=False, block_threads=0):
+                               coriolis_const=False, unfused=False, linear_blocks=False, block_threads=0,
+                               general_tiles=False):
     """Factory mirroring create_linear_swm_operator
     (operator_swm_factory_mod.f90 L586-685) for the Ch staggering.
     unfused=True selects the multi-kernel diagnostic path; linear_blocks=True
-    the plain block order of the fused stage kernel; block_threads (256 or
-    512, 0 = default) its workgroup size."""
+    the plain block order of the fused stage kernel; general_tiles=True sends
+    every tile through its general path; block_threads (0 or 512) its
+    workgroup size."""
     return OperatorSwmLin(Config(N, order, precision, radius, grav, h_mean, omega, axis,
-                                 coriolis_const, unfused, linear_blocks, block_threads))
+                                 coriolis_const, unfused, linear_blocks, block_threads, general_tiles))
 
 
 class Ensemble:

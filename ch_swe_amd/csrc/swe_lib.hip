@@ -574,6 +574,7 @@ struct Engine : EngineBase {
         a.seg_desc = seg_desc;
         a.order = O;
         a.xcd = (pl->cfg.flags & SWE_FLAG_LINEAR_BLOCKS) ? 0 : 1;
+        a.general = (pl->cfg.flags & SWE_FLAG_GENERAL_TILES) ? 1 : 0;
         return a;
     }
 
@@ -786,8 +787,8 @@ int swe_plan_create(const swe_config *cfg, swe_plan **out)
     if (cfg->N > 4096) return fail(SWE_EINVAL, "N above 4096 is not supported");
     if (cfg->precision != SWE_FP64 && cfg->precision != SWE_FP32)
         return fail(SWE_EINVAL, "precision must be SWE_FP64 or SWE_FP32");
-    if (cfg->block_threads != 0 && cfg->block_threads != 256 && cfg->block_threads != 512)
-        return fail(SWE_EINVAL, "block_threads must be 0, 256 or 512");
+    if (cfg->block_threads != 0 && cfg->block_threads != F_THREADS)
+        return fail(SWE_EINVAL, "block_threads must be 0 or 512 (the fused kernels' workgroup size)");
     if (!(cfg->radius > 0) || !std::isfinite(cfg->grav) || !std::isfinite(cfg->h_mean) ||
         !std::isfinite(cfg->omega))
         return fail(SWE_EINVAL, "non-finite or non-positive physical constant");

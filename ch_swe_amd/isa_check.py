@@ -9,6 +9,9 @@ address on some path.  This has happened with this toolchain: a stage kernel
 that selected its store target from a runtime mode switch stored the last
 RK4 stage through an SGPR pair that was only written on the other modes'
 paths, and faulted on the GPU.  The build refuses a library with any finding.
+
+Paths that cannot run are left out where the assembly shows it plainly: a
+branch on an exec mask that was just set to zero (prune_zero_exec_edges).
 """
 import re
 import sys
@@ -50,6 +53,61 @@ def split_ops(rest):
     if cur.strip():
         parts.append(cur.strip())
     return parts
+
+
+def _kills(mn, ops):
+    if mn != '__kill__':
+        return set()
+    return set(int(x) for x in re.findall(r'sgpr(\d+)', ops[0]))
+
+
+def _defs(mn, ops):
+    if mn == '__kill__' or not ops or NO_DEST.match(mn):
+        return set()
+    d = set(regs(ops[0]))
+    if VOP3B.match(mn) and len(ops) > 1:
+        d.update(regs(ops[1]))
+    return d
+
+
+def prune_zero_exec_edges(blocks, labels, preds):
+    """Drop edges that cannot run: a block made of `s_and_saveexec_b64 d, m`
+    and `s_cbranch_execz L`, entered from a predecessor where m is known to be
+    zero, sets exec to zero and always branches to L.  For each such
+    predecessor the block is cloned with L as its only successor.  Known-zero
+    SGPRs: a must-analysis over the CFG (`s_mov_b32/b64 s, 0` makes s zero,
+    any other write or implicit-def unmakes it)."""
+    ALL = set(range(0, 106))
+    zout = [set(ALL) for _ in blocks]
+    changed = True
+    while changed:
+        changed = False
+        for i, b in enumerate(blocks):
+            ps = [zout[p] for p in preds[i]]
+            z = set() if i == 0 else (set.intersection(*ps) if ps else set(ALL))
+            for ln, mn, ops in b['ins']:
+                z -= _kills(mn, ops)
+                d = _defs(mn, ops)
+                z -= d
+                if mn in ('s_mov_b32', 's_mov_b64') and len(ops) == 2 and ops[1] == '0':
+                    z |= d
+            if z != zout[i]:
+                zout[i] = z
+                changed = True
+    for i in range(len(blocks)):
+        ins = blocks[i]['ins']
+        if (len(ins) != 2 or ins[0][1] != 's_and_saveexec_b64' or len(ins[0][2]) != 2
+                or ins[1][1] != 's_cbranch_execz'):
+            continue
+        mask = set(regs(ins[0][2][1]))
+        if not mask:
+            continue
+        target = labels[ins[1][2][0]]
+        for p in list(preds[i]):
+            if p != i and mask <= zout[p]:
+                clone = {'label': blocks[i]['label'] + f'/exec0-from-{p}', 'ins': list(ins), 'succ': [target]}
+                blocks.append(clone)
+                blocks[p]['succ'] = [len(blocks) - 1 if s == i else s for s in blocks[p]['succ']]
 
 
 def analyze(name, body, n_entry):
@@ -95,6 +153,11 @@ def analyze(name, body, n_entry):
             b['succ'] = []
         else:
             b['succ'] = [i + 1] if i + 1 < len(blocks) else []
+    preds = [[] for _ in blocks]
+    for i, b in enumerate(blocks):
+        for s in b['succ']:
+            preds[s].append(i)
+    prune_zero_exec_edges(blocks, labels, preds)
     preds = [[] for _ in blocks]
     for i, b in enumerate(blocks):
         for s in b['succ']:

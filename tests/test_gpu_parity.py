@@ -61,15 +61,15 @@ def test_apply_fp64_matches_oracle(oracle_mod, order, N, cor_const):
         assert _rel(g, r) < 1e-12
 
 
-@pytest.mark.parametrize("block_threads,linear", [(256, False), (512, True)])
-def test_launch_variants_match_oracle(oracle_mod, block_threads, linear):
-    """Workgroup size 256 (every tile on the general path) and the plain
-    block order give the same operator."""
+@pytest.mark.parametrize("general,linear", [(True, False), (False, True)])
+def test_launch_variants_match_oracle(oracle_mod, general, linear):
+    """Every tile on the general path, and the plain block order, give the
+    same operator."""
     N, order = 80, 42
-    _say(f"apply fp64 N={N} block_threads={block_threads} linear={linear}")
+    _say(f"apply fp64 N={N} general_tiles={general} linear={linear}")
     o = oracle_mod.Oracle(N, order)
     u, v, h = _random_state(o, 5)
-    op = _op(N, order, block_threads=block_threads, linear_blocks=linear)
+    op = _op(N, order, general_tiles=general, linear_blocks=linear)
     vin, vout = op.create_stvec(), op.create_stvec()
     vin.upload(u, v, h)
     op.apply(vout, vin)

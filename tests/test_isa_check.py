@@ -56,3 +56,41 @@ def test_built_library_assembly_is_clean():
 
         pytest.skip("library not built in this tree")
     assert isa_check.check_file(asm) == {}
+
+
+# .LBB0_3 (the store) is reached from .LBB0_2, where s[28:29] is defined, and
+# from .LBB0_1 only through a saveexec of a mask set to zero just before: that
+# path always takes the execz branch to .LBB0_4 and never stores
+ZERO_EXEC = """
+_Z4kernv: ; @_Z4kernv
+; %bb.0:
+\ts_load_dwordx4 s[8:11], s[0:1], 0x0
+\ts_cmp_eq_u32 s2, 3
+\ts_cbranch_scc1 .LBB0_2
+; %bb.1:
+\ts_mov_b64 s[14:15], 0
+                                        ; implicit-def: $sgpr28_sgpr29
+\ts_branch .LBB0_5
+.LBB0_2:
+\ts_mov_b64 s[28:29], s[8:9]
+\ts_mov_b64 s[14:15], exec
+.LBB0_5:
+\ts_and_saveexec_b64 s[0:1], s[14:15]
+\ts_cbranch_execz .LBB0_4
+.LBB0_3:
+\tv_lshl_add_u64 v[4:5], v[2:3], 3, s[28:29]
+\tglobal_store_dwordx2 v[4:5], v[6:7], off
+.LBB0_4:
+\ts_endpgm
+.Lfunc_end0:
+"""
+
+
+def test_branch_on_zeroed_exec_mask_is_not_a_path(tmp_path):
+    assert _check(ZERO_EXEC, tmp_path) == {}
+
+
+def test_nonzero_mask_keeps_the_path(tmp_path):
+    live = ZERO_EXEC.replace("\ts_mov_b64 s[14:15], 0\n", "\ts_mov_b64 s[14:15], exec\n")
+    found = _check(live, tmp_path)
+    assert {r[3] for r in found["_Z4kernv"]} == {28, 29}
