@@ -532,14 +532,14 @@ struct Engine : EngineBase {
         cf = Coef<T>{tabs[0], tabs[1], tabs[2], tabs[3], tabs[4], tabs[5], tabs[6], tabs[7], tf, qb, qt, ql, qr};
         for (auto &r : rec)
             if ((rc = alloc((void **)&r, (size_t)24 * REC_KINDS * n * sizeof(T)))) return rc;
-        // cube side table (constant memory, identical for every plan)
-        SideDesc sd[6][4];
+        // the kernels' packed cube side table must match the topology
         for (int q = 0; q < 6; q++)
             for (int sdi = 0; sdi < 4; sdi++) {
-                swe::SideMatch m = swe::match_side(q, sdi);
-                sd[q][sdi] = SideDesc{m.q, m.q_side, m.reversed ? 1 : 0};
+                const swe::SideMatch m = swe::match_side(q, sdi);
+                const SideDesc k = side_of(q, sdi);
+                if (k.q != m.q || k.qs != m.q_side || k.rev != (m.reversed ? 1 : 0))
+                    return fail(SWE_EINVAL, "internal: packed cube side table disagrees with the topology");
             }
-        HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_side), sd, sizeof(sd)));
         // tiles, edge segments and which tiles write into each segment's band
         FusedTables ft;
         std::string err;
