@@ -101,6 +101,17 @@ SWE_HD T d_i2c_pt(const A &f, int s, int n, int k)
     }
 }
 
+// sbp42 centres -> interfaces closure: rows 1..4 (Q), the SAT weight C and
+// the extrapolation R of the neighbour's normal flux
+struct Sbp42C2i {
+    static constexpr double Q[4][5] = {{-2.0, 3.0, -1.0, 0, 0},
+                                       {-1.0, 1.0, 0, 0, 0},
+                                       {1.0 / 24, -9.0 / 8, 9.0 / 8, -1.0 / 24, 0},
+                                       {-1.0 / 71, 6.0 / 71, -83.0 / 71, 81.0 / 71, -3.0 / 71}};
+    static constexpr double C = 0.5 / (7.0 / 18);
+    static constexpr double R[3] = {15.0 / 8, -5.0 / 4, 3.0 / 8};
+};
+
 // ---- centres -> interfaces derivative (+ SAT with neighbour's normal flux
 //      at depths 1..ORDER/21+1 stored in f[0], f[-1], ... and f[n+1], ...)
 template <int ORDER, typename T, typename A>
@@ -119,12 +130,9 @@ SWE_HD T d_c2i_pt(const A &f, int s, int n, int k)
             v -= T(1.5) * (ld(f, s, nx - 1) - ld(f, s, nx)) - T(0.5) * (ld(f, s, nx - 2) - ld(f, s, nx + 1));
         return v;
     } else {
-        constexpr double Q[4][5] = {{-2.0, 3.0, -1.0, 0, 0},
-                                    {-1.0, 1.0, 0, 0, 0},
-                                    {1.0 / 24, -9.0 / 8, 9.0 / 8, -1.0 / 24, 0},
-                                    {-1.0 / 71, 6.0 / 71, -83.0 / 71, 81.0 / 71, -3.0 / 71}};
-        constexpr double C = 0.5 / (7.0 / 18);
-        constexpr double R[3] = {15.0 / 8, -5.0 / 4, 3.0 / 8};
+        const auto &Q = Sbp42C2i::Q;
+        constexpr double C = Sbp42C2i::C;
+        const auto &R = Sbp42C2i::R;
         if (k <= 4) {
             const int r = k - 1;
             T v = T(Q[r][0]) * ld(f, s, 1) + T(Q[r][1]) * ld(f, s, 2) + T(Q[r][2]) * ld(f, s, 3) +
@@ -145,6 +153,57 @@ SWE_HD T d_c2i_pt(const A &f, int s, int n, int k)
             return v;
         }
         return d_c2i_in<42, T>([&](int kk) { return ld(f, s, kk); }, k);
+    }
+}
+
+// d_c2i_pt at row k on a window of the line held in registers: w[j] = f(ws +
+// j), ws = c2i_window_start<ORDER>(k, n).  sbp21: 4 values (rows 1 and n + 1
+// read the two end points and two SAT values, inner rows k - 1, k); sbp42: 8
+// values (closure rows read the first / last five points and three SAT
+// values, inner rows k - 2 .. k + 1).  The same arithmetic as d_c2i_pt, with
+// a constant window index in every branch.
+template <int ORDER>
+SWE_HD int c2i_window_start(int k, int n)
+{
+    const int nx = n + 1;
+    if (ORDER == 21) return k == 1 ? -1 : (k == nx ? nx - 2 : k - 1);
+    return k <= 4 ? -2 : (k >= nx - 3 ? nx - 5 : k - 2);
+}
+template <int ORDER, typename T>
+SWE_HD T d_c2i_win(const T *w, int k, int n)
+{
+    const int nx = n + 1;
+    if (ORDER == 21) {
+        if (k == 1) {
+            T v = w[3] - w[2];
+            v += T(1.5) * (w[2] - w[1]) - T(0.5) * (w[3] - w[0]);
+            return v;
+        }
+        if (k == nx) {
+            T v = w[1] - w[0];
+            v -= T(1.5) * (w[1] - w[2]) - T(0.5) * (w[0] - w[3]);
+            return v;
+        }
+        return w[1] - w[0];
+    } else {
+        const auto &Q = Sbp42C2i::Q;
+        constexpr double C = Sbp42C2i::C;
+        const auto &R = Sbp42C2i::R;
+        if (k <= 4) {  // w[j] = f(j - 2)
+            const int r = k - 1;
+            T v = T(Q[r][0]) * w[3] + T(Q[r][1]) * w[4] + T(Q[r][2]) * w[5] + T(Q[r][3]) * w[6] + T(Q[r][4]) * w[7];
+            if (k == 1) v += T(C) * (T(R[0]) * (w[3] - w[2]) + T(R[1]) * (w[4] - w[1]) + T(R[2]) * (w[5] - w[0]));
+            return v;
+        }
+        if (k >= nx - 3) {  // w[j] = f(nx - 5 + j)
+            const int r = nx - k;
+            T v = -(T(Q[r][0]) * w[4] + T(Q[r][1]) * w[3] + T(Q[r][2]) * w[2] + T(Q[r][3]) * w[1] +
+                    T(Q[r][4]) * w[0]);
+            if (k == nx) v -= T(C) * (T(R[0]) * (w[4] - w[5]) + T(R[1]) * (w[3] - w[6]) + T(R[2]) * (w[2] - w[7]));
+            return v;
+        }
+        // w[j] = f(k - 2 + j)
+        return (T(27) * (w[2] - w[1]) - (w[3] - w[0])) * T(1.0 / 24);
     }
 }
 

@@ -70,13 +70,25 @@ def _defs(mn, ops):
     return d
 
 
+def _zero_step(z, mn, ops):
+    """Known-zero SGPRs after one instruction."""
+    z = z - _kills(mn, ops)
+    d = _defs(mn, ops)
+    z = z - d
+    if mn in ('s_mov_b32', 's_mov_b64') and len(ops) == 2 and ops[1] == '0':
+        z = z | d
+    return z
+
+
 def prune_zero_exec_edges(blocks, labels, preds):
-    """Drop edges that cannot run: a block made of `s_and_saveexec_b64 d, m`
-    and `s_cbranch_execz L`, entered from a predecessor where m is known to be
-    zero, sets exec to zero and always branches to L.  For each such
-    predecessor the block is cloned with L as its only successor.  Known-zero
-    SGPRs: a must-analysis over the CFG (`s_mov_b32/b64 s, 0` makes s zero,
-    any other write or implicit-def unmakes it)."""
+    """Drop edges that cannot run.  A block whose last `s_and_saveexec_b64 d,
+    m` is followed by nothing that writes exec up to its final
+    `s_cbranch_execz L` / `s_cbranch_execnz L`, entered from a predecessor
+    after which m is known to be zero, runs its branch with exec zero: execz
+    always goes to L, execnz never does.  For each such predecessor the block
+    is cloned with that one successor.  Known-zero SGPRs: a must-analysis over
+    the CFG (`s_mov_b32/b64 s, 0` makes s zero, any other write or
+    implicit-def unmakes it)."""
     ALL = set(range(0, 106))
     zout = [set(ALL) for _ in blocks]
     changed = True
@@ -86,26 +98,40 @@ def prune_zero_exec_edges(blocks, labels, preds):
             ps = [zout[p] for p in preds[i]]
             z = set() if i == 0 else (set.intersection(*ps) if ps else set(ALL))
             for ln, mn, ops in b['ins']:
-                z -= _kills(mn, ops)
-                d = _defs(mn, ops)
-                z -= d
-                if mn in ('s_mov_b32', 's_mov_b64') and len(ops) == 2 and ops[1] == '0':
-                    z |= d
+                z = _zero_step(z, mn, ops)
             if z != zout[i]:
                 zout[i] = z
                 changed = True
-    for i in range(len(blocks)):
+
+    def writes_exec(mn, ops):
+        return mn.startswith('v_cmpx') or 'saveexec' in mn or (ops and ops[0] in ('exec', 'exec_lo', 'exec_hi'))
+
+    n0 = len(blocks)
+    for i in range(n0):
         ins = blocks[i]['ins']
-        if (len(ins) != 2 or ins[0][1] != 's_and_saveexec_b64' or len(ins[0][2]) != 2
-                or ins[1][1] != 's_cbranch_execz'):
+        if not ins or ins[-1][1] not in ('s_cbranch_execz', 's_cbranch_execnz'):
             continue
-        mask = set(regs(ins[0][2][1]))
+        k = len(ins) - 2
+        while k >= 0 and not writes_exec(ins[k][1], ins[k][2]):
+            k -= 1
+        if k < 0 or ins[k][1] != 's_and_saveexec_b64' or len(ins[k][2]) != 2:
+            continue
+        mask = set(regs(ins[k][2][1]))
         if not mask:
             continue
-        target = labels[ins[1][2][0]]
+        taken = labels[ins[-1][2][0]]
+        fall = i + 1 if i + 1 < n0 else None
+        only = taken if ins[-1][1] == 's_cbranch_execz' else fall
+        if only is None:
+            continue
         for p in list(preds[i]):
-            if p != i and mask <= zout[p]:
-                clone = {'label': blocks[i]['label'] + f'/exec0-from-{p}', 'ins': list(ins), 'succ': [target]}
+            if p == i:
+                continue
+            z = set(zout[p])
+            for ln, mn, ops in ins[:k]:
+                z = _zero_step(z, mn, ops)
+            if mask <= z:
+                clone = {'label': blocks[i]['label'] + f'/exec0-from-{p}', 'ins': list(ins), 'succ': [only]}
                 blocks.append(clone)
                 blocks[p]['succ'] = [len(blocks) - 1 if s == i else s for s in blocks[p]['succ']]
 

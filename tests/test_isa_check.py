@@ -94,3 +94,49 @@ def test_nonzero_mask_keeps_the_path(tmp_path):
     live = ZERO_EXEC.replace("\ts_mov_b64 s[14:15], 0\n", "\ts_mov_b64 s[14:15], exec\n")
     found = _check(live, tmp_path)
     assert {r[3] for r in found["_Z4kernv"]} == {28, 29}
+
+
+def test_zeroed_mask_with_scalar_work_before_the_branch(tmp_path):
+    """Instructions between the saveexec and the execz branch that do not
+    write exec keep the branch taken."""
+    txt = ZERO_EXEC.replace("\ts_and_saveexec_b64 s[0:1], s[14:15]\n",
+                            "\ts_and_saveexec_b64 s[0:1], s[14:15]\n\tv_readlane_b32 s4, v7, 4\n")
+    assert _check(txt, tmp_path) == {}
+    bad = txt.replace("\tv_readlane_b32 s4, v7, 4\n", "\ts_mov_b64 exec, s[0:1]\n")
+    assert {r[3] for r in _check(bad, tmp_path)["_Z4kernv"]} == {28, 29}
+
+
+# the store block is entered by execnz on a lane mask that only the block
+# defining the store base sets; on the path that skips that block the mask is
+# still the zero it was given, so the execnz branch is not taken
+EXECNZ = """
+_Z4kernv: ; @_Z4kernv
+; %bb.0:
+\ts_load_dwordx4 s[8:11], s[0:1], 0x0
+\ts_mov_b64 s[20:21], 0
+\tv_cmp_gt_u32_e32 vcc, 7, v0
+                                        ; implicit-def: $sgpr18_sgpr19
+\ts_and_saveexec_b64 s[2:3], vcc
+\ts_cbranch_execz .LBB0_2
+; %bb.1:
+\ts_mov_b64 s[18:19], s[8:9]
+\ts_or_b64 s[20:21], s[20:21], exec
+.LBB0_2:
+\ts_or_b64 exec, exec, s[2:3]
+\ts_and_saveexec_b64 s[0:1], s[20:21]
+\ts_cbranch_execnz .LBB0_3
+; %bb.4:
+\ts_endpgm
+.LBB0_3:
+\tv_lshl_add_u64 v[4:5], s[18:19], 3, v[2:3]
+\tglobal_store_dwordx2 v[4:5], v[6:7], off
+\ts_endpgm
+.Lfunc_end0:
+"""
+
+
+def test_execnz_on_a_mask_still_zero_is_not_a_path(tmp_path):
+    assert _check(EXECNZ, tmp_path) == {}
+    # a mask that is not known to be zero on the skip path keeps the finding
+    live = EXECNZ.replace("\ts_mov_b64 s[20:21], 0\n", "\ts_mov_b64 s[20:21], exec\n")
+    assert {r[3] for r in _check(live, tmp_path)["_Z4kernv"]} == {18, 19}
