@@ -89,7 +89,30 @@ def run_gpu(cs, precision, N, state0, dt, steps, warmup, ev):
     rk4.step(scratch, op, dt, min(steps, 200))
     out["stage_ms"], out["n_stage"], out["records_ms"], out["n_records"] = op.kernel_times()
     op.kernel_timing(False)
+    # what an event pair adds around a launch: an empty pair on the same
+    # stream, back to back (median of 50), subtracted from the per-launch
+    # durations above
+    op.synchronize()
+    gaps = []
+    for _ in range(50):
+        ev.record(a, stream)
+        ev.record(b, stream)
+        gaps.append(ev.elapsed_ms(a, b))
+    out["event_pair_ms"] = float(np.median(gaps))
     return out
+
+
+TRAFFIC_FILE = "stage_traffic_N384_sbp42_fp32.json"
+
+
+def measured_traffic(N, dtype):
+    """HBM bytes per stage launch from the committed counter runs of this
+    workload (N=384 sbp42 FP32), or None for any other workload."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", TRAFFIC_FILE)
+    if N != 384 or ORDER != 42 or dtype != "f32" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def rel_l2(a, b, w):
@@ -382,9 +405,12 @@ def main():
     # the increment.)  The timed region also holds the edge-record launches
     # (one after each stage).  Durations come from the event-bracketed pass.
     launch_bytes = 4 * dof(N) * 4
-    launch_us = r32["stage_ms"] * 1e3 / r32["n_stage"]
-    records_us = r32["records_ms"] * 1e3 / max(r32["n_records"], 1)
+    pair_us = r32["event_pair_ms"] * 1e3
+    launch_raw_us = r32["stage_ms"] * 1e3 / r32["n_stage"]
+    launch_us = launch_raw_us - pair_us
+    records_us = r32["records_ms"] * 1e3 / max(r32["n_records"], 1) - pair_us
     achieved = launch_bytes / (launch_us * 1e-6) / 1e9
+    traffic = measured_traffic(N, "f32")
 
     errs = None
     if r64 is not None:
@@ -423,13 +449,18 @@ def main():
             "kernel": "k_stage (fused RK4 stage), mean of the 4 stage launches per step",
             "bytes_per_launch": launch_bytes,
             "us_per_launch": round(launch_us, 3),
+            "us_per_launch_event_pairs": round(launch_raw_us, 3),
+            "event_pair_overhead_us": round(pair_us, 3),
             "records_us_per_launch": round(records_us, 3),
             "step_fraction_in_stage_kernel": round(4 * launch_us * 1e-3 / ms_step, 3),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": None if traffic is None else traffic["hbm_bytes_per_launch_mean"],
+            "traffic_source": None if traffic is None else
+            "rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 512 MiB copy), mean of the stage "
+            "launches of a step, profiles/" + TRAFFIC_FILE,
         },
         "cpu_baseline": {
             "value": None if cpu_val is None else round(cpu_val, 3),

@@ -429,7 +429,6 @@ struct Engine : EngineBase {
     T *rec[2] = {nullptr, nullptr};
     int ntx = 0, nty = 0, ntiles = 0, nseg = 0, fwx = 0, fwy = 0;
     int nthr = F_THREADS;  // workgroup size of the fused kernels
-    int4 *seg_desc = nullptr;
 
     size_t elem() const override { return sizeof(T); }
     bool fused() const override { return use_fused; }
@@ -551,8 +550,12 @@ struct Engine : EngineBase {
         ntiles = ft.ntiles;
         nseg = ft.nseg;
         const std::vector<int4> &segs = ft.segs;
-        if ((rc = alloc((void **)&seg_desc, segs.size() * sizeof(int4)))) return rc;
-        HIP_TRY(hipMemcpyAsync(seg_desc, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice, pl->stream));
+        // the records kernels decode their segment from the block index
+        for (int i = 0; i < nseg; i++) {
+            const int4 a = segment_of<TY>(N, i), b = segs[i];
+            if (a.x != b.x || a.y != b.y || a.z != b.z || a.w != b.w)
+                return fail(SWE_EINVAL, "internal: edge segment decode disagrees with the segment table");
+        }
         HIP_TRY(hipStreamSynchronize(pl->stream));
         return SWE_OK;
     }
@@ -571,7 +574,6 @@ struct Engine : EngineBase {
         a.nty = nty;
         a.fwx = fwx;
         a.fwy = fwy;
-        a.seg_desc = seg_desc;
         a.order = O;
         a.xcd = (pl->cfg.flags & SWE_FLAG_LINEAR_BLOCKS) ? 0 : 1;
         a.general = (pl->cfg.flags & SWE_FLAG_GENERAL_TILES) ? 1 : 0;
@@ -582,7 +584,7 @@ struct Engine : EngineBase {
     void launch_records(const T *y, T *r)
     {
         hipEvent_t e = timer.begin(pl->stream);
-        k_records<T, O, TY><<<nseg, nthr, 0, pl->stream>>>(d, cf, y, r, seg_desc);
+        k_records<T, O, TY><<<nseg, nthr, 0, pl->stream>>>(d, cf, y, r);
         timer.end(e, pl->stream, 1);
     }
 
