@@ -82,23 +82,20 @@ def run_gpu(cs, precision, N, state0, dt, steps, warmup, ev):
     ms = ev.elapsed_ms(a, b)
     m1, e1 = op.integrals(st)
     out = dict(ms=ms, wall=wall, state=st.download(), energy_drift=e1 / e0 - 1.0)
-    # per-kernel durations from a separate pass with every stage / records
-    # launch bracketed by HIP events on the plan's stream
+    # per-kernel durations from a separate pass in which every stage /
+    # records launch carries dispatch-timestamped start and stop events
     scratch = op.create_stvec().upload(*out["state"])
     op.kernel_timing(True)
     rk4.step(scratch, op, dt, min(steps, 200))
     out["stage_ms"], out["n_stage"], out["records_ms"], out["n_records"] = op.kernel_times()
     op.kernel_timing(False)
-    # what an event pair adds around a launch: an empty pair on the same
-    # stream, back to back (median of 50), subtracted from the per-launch
-    # durations above
-    op.synchronize()
-    gaps = []
-    for _ in range(50):
-        ev.record(a, stream)
-        ev.record(b, stream)
-        gaps.append(ev.elapsed_ms(a, b))
-    out["event_pair_ms"] = float(np.median(gaps))
+    # host-boundary rate: upload of the FP64 host state, the steps, download
+    # (what a caller handing over host buffers sees), outside the timed region
+    t0 = time.perf_counter()
+    st2 = op.create_stvec().upload(*state0)
+    rk4.step(st2, op, dt, steps)
+    st2.download()
+    out["host_boundary_s"] = time.perf_counter() - t0
     return out
 
 
@@ -403,12 +400,11 @@ def main():
     # per launch on average, one vector = DOF * 4 bytes.  (The kernels move
     # 17 vectors a step: the last stage also reads v0, because acc holds only
     # the increment.)  The timed region also holds the edge-record launches
-    # (one after each stage).  Durations come from the event-bracketed pass.
+    # (one after each stage).  Durations come from the pass with
+    # dispatch-timestamped events (the kernels' own run times).
     launch_bytes = 4 * dof(N) * 4
-    pair_us = r32["event_pair_ms"] * 1e3
-    launch_raw_us = r32["stage_ms"] * 1e3 / r32["n_stage"]
-    launch_us = launch_raw_us - pair_us
-    records_us = r32["records_ms"] * 1e3 / max(r32["n_records"], 1) - pair_us
+    launch_us = r32["stage_ms"] * 1e3 / r32["n_stage"]
+    records_us = r32["records_ms"] * 1e3 / max(r32["n_records"], 1)
     achieved = launch_bytes / (launch_us * 1e-6) / 1e9
     traffic = measured_traffic(N, "f32")
 
@@ -436,6 +432,7 @@ def main():
         "data": "synthetic (seeded degree-32 spherical-harmonic h0, 20 m RMS, balanced winds)",
         "config": {"workload": f"single large solve: N={N} (6x{N}^2 cells), Ch, sbp42, RK4 at CFL 0.5",
                    "N": N, "order": ORDER, "dt_s": round(dt, 4), "cells": cells, "dof": dof(N)},
+        "host_boundary_value": round(cells * args.steps / r32["host_boundary_s"] / 1e6, 2),
         "fp64": None if r64 is None else {
             "value": round(cells * args.steps / (r64["ms"] * 1e-3) / 1e6, 2),
             "ms_per_step": round(r64["ms"] / args.steps, 5)},
@@ -449,8 +446,6 @@ def main():
             "kernel": "k_stage (fused RK4 stage), mean of the 4 stage launches per step",
             "bytes_per_launch": launch_bytes,
             "us_per_launch": round(launch_us, 3),
-            "us_per_launch_event_pairs": round(launch_raw_us, 3),
-            "event_pair_overhead_us": round(pair_us, 3),
             "records_us_per_launch": round(records_us, 3),
             "step_fraction_in_stage_kernel": round(4 * launch_us * 1e-3 / ms_step, 3),
             "achieved": round(achieved, 1),

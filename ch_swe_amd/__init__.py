@@ -241,7 +241,8 @@ class OperatorSwmLin:
         return lib().swe_plan_stream(self.plan.h)
 
     def kernel_timing(self, enable=True):
-        """Bracket every fused stage / records launch with HIP events."""
+        """Time every fused stage / records launch (dispatch-timestamped
+        start and stop events)."""
         _check(lib().swe_plan_timing(self.plan.h, 1 if enable else 0))
 
     def kernel_times(self):

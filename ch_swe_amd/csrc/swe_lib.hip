@@ -20,6 +20,7 @@
 // plan-time maps (swe_topology.cpp).  Fields live in halo-padded per-panel
 // arrays (HALO = 4 on each side, common pitch for all three meshes).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <deque>
@@ -313,8 +314,10 @@ struct DevBuf {
     }
 };
 
-// Optional per-launch timing of the fused kernels (measurement aid): HIP
-// events recorded on the plan stream around every stage and record launch.
+// Optional per-launch timing of the fused kernels (measurement aid): each
+// stage and records launch goes out through hipExtLaunchKernelGGL with a
+// start and a stop event, which the dispatch itself timestamps (no marker
+// packets between the kernels, so the spans are the kernels' own run times).
 struct KernelTimer {
     bool on = false;
     std::vector<hipEvent_t> pool;
@@ -342,19 +345,17 @@ struct KernelTimer {
         (void)hipEventCreate(&e);
         return e;
     }
-    // bracket one launch: begin() before, end() after
-    hipEvent_t begin(hipStream_t s)
+    // launch kernel on grid x nthr threads, timed when on (kind 0: stage
+    // kernel, 1: records kernel)
+    template <class K, class... A>
+    void launch(K kernel, int grid, int nthr, hipStream_t s, int kind, A... args)
     {
-        if (!on) return nullptr;
-        hipEvent_t a = get();
-        (void)hipEventRecord(a, s);
-        return a;
-    }
-    void end(hipEvent_t a, hipStream_t s, int kind)
-    {
-        if (!on || !a) return;
-        hipEvent_t b = get();
-        (void)hipEventRecord(b, s);
+        if (!on) {
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(nthr), 0, s, args...);
+            return;
+        }
+        hipEvent_t a = get(), b = get();
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(nthr), 0, s, a, b, 0, args...);
         spans.push_back(Span{a, b, kind});
     }
     int collect(double *ms, int *cnt)
@@ -583,9 +584,7 @@ struct Engine : EngineBase {
     template <int O>
     void launch_records(const T *y, T *r)
     {
-        hipEvent_t e = timer.begin(pl->stream);
-        k_records<T, O, TY><<<nseg, nthr, 0, pl->stream>>>(d, cf, y, r);
-        timer.end(e, pl->stream, 1);
+        timer.launch(k_records<T, O, TY>, nseg, nthr, pl->stream, 1, d, cf, y, r);
     }
 
     template <int O>
@@ -595,9 +594,7 @@ struct Engine : EngineBase {
         StageArgs<T> a = stage_args<O>(in);
         a.out = out;
         a.rec_in = rec[0];
-        hipEvent_t e = timer.begin(pl->stream);
-        k_stage<T, O, TY, 0><<<ntiles, nthr, 0, pl->stream>>>(a);
-        timer.end(e, pl->stream, 0);
+        timer.launch(k_stage<T, O, TY, 0>, ntiles, nthr, pl->stream, 0, a);
     }
 
     template <int O>
@@ -618,11 +615,9 @@ struct Engine : EngineBase {
                 a.a_dt = T(A[st]);
                 a.b_dt = T(B[st]);
                 a.rec_in = rec[st & 1];
-                hipEvent_t e = timer.begin(pl->stream);
-                if (st == 0) k_stage<T, O, TY, 1><<<ntiles, nthr, 0, pl->stream>>>(a);
-                else if (st == 3) k_stage<T, O, TY, 3><<<ntiles, nthr, 0, pl->stream>>>(a);
-                else k_stage<T, O, TY, 2><<<ntiles, nthr, 0, pl->stream>>>(a);
-                timer.end(e, pl->stream, 0);
+                if (st == 0) timer.launch(k_stage<T, O, TY, 1>, ntiles, nthr, pl->stream, 0, a);
+                else if (st == 3) timer.launch(k_stage<T, O, TY, 3>, ntiles, nthr, pl->stream, 0, a);
+                else timer.launch(k_stage<T, O, TY, 2>, ntiles, nthr, pl->stream, 0, a);
                 // records of the stage output for the next stage (none after
                 // the last stage of the last step)
                 if (st < 3) launch_records<O>(nexts[st], rec[(st + 1) & 1]);

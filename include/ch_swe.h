@@ -133,7 +133,8 @@ int swe_server_stats(swe_server *srv, int *queued, int *active, size_t *bytes_in
 
 /* Measurement aid (no counterpart in the reference): with timing enabled,
  * every launch of the fused stage kernel and of the edge-record kernel on
- * this plan is bracketed by HIP events on the plan's stream.
+ * this plan carries start / stop HIP events timestamped by the dispatch
+ * itself (the kernels' own run times).
  * swe_plan_kernel_times waits for them, returns the summed durations (ms)
  * and counts since the last call, and starts a new interval.  Enabling or
  * disabling drops what was recorded. */
