@@ -415,6 +415,9 @@ def main():
 
     cpu_val, cpu_sec, cpu_threads = (cpu_baseline(N, state0, dt, args.cpu_steps, args.cpu_threads)
                                      if args.cpu_steps > 0 else (None, 0.0, 0))
+    # the same oracle on one core, a tenth of the steps
+    one_steps = max(1, args.cpu_steps // 10)
+    cpu1_val, cpu1_sec, _ = (cpu_baseline(N, state0, dt, one_steps, 1) if args.cpu_steps > 0 else (None, 0.0, 0))
 
     line = {
         "metric": "Mpts*steps/s (FP32 stencils, FP64 metric and integrals), N=384 sbp42 single solve",
@@ -464,6 +467,8 @@ def main():
             "kind": "port",
             "sample": f"CPU oracle (C, FP64, OpenMP row loops, {cpu_threads} threads), same N={N} "
                       f"sbp42 state, {args.cpu_steps} RK4 steps ({cpu_sec:.1f} s)",
+            "single_core_value": None if cpu1_val is None else round(cpu1_val, 3),
+            "single_core_sample": f"the same oracle on 1 thread, {one_steps} RK4 steps ({cpu1_sec:.1f} s)",
         },
     }
     print(json.dumps(line), flush=True)

@@ -13,7 +13,7 @@
 !
 ! The abstract types below restate the reference's deferred bindings that this
 ! path uses; a build inside the reference extends the reference's own
-! operator_t / stvec_t / timescheme_t instead (INTEGRATION.md).
+! operator_t / stvec_t / timescheme_t instead (same procedures, same calls).
 module swe_hip_mod
 
     use iso_c_binding, only : c_ptr, c_null_ptr, c_associated, c_int, c_double
