@@ -42,6 +42,7 @@ EXPORTS = (
     "swe_ensemble_destroy", "swe_ensemble_rk4", "swe_ensemble_stream", "swe_ensemble_synchronize",
     "swe_plan_timing", "swe_plan_kernel_times", "swe_server_create", "swe_server_destroy",
     "swe_server_submit", "swe_server_tick", "swe_server_take", "swe_server_stats",
+    "swe_check_tables",
 )
 
 
@@ -369,6 +370,12 @@ def create_timescheme(name):
     if name != "rk4":
         raise ValueError(f"unknown time scheme '{name}' (this build provides 'rk4')")
     return Rk4()
+
+
+def check_tables(N, precision=FP32):
+    """Host-only self-check of the fused kernels' plan-time tables (see
+    swe_check_tables); raises SweError on a mismatch."""
+    _check(lib().swe_check_tables(int(N), int(precision)))
 
 
 def halo_map(N, order, kind):

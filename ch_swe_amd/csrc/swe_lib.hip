@@ -991,5 +991,51 @@ int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *
 }
 
 }  // extern "C"
+
+namespace {
+template <int TY>
+int check_tables_ty(int N)
+{
+    for (int q = 0; q < 6; q++)
+        for (int sd = 0; sd < 4; sd++) {
+            const swe::SideMatch m = swe::match_side(q, sd);
+            const SideDesc k = side_of(q, sd);
+            if (k.q != m.q || k.qs != m.q_side || k.rev != (m.reversed ? 1 : 0))
+                return fail(SWE_EINVAL, "packed cube side table disagrees with the topology");
+        }
+    FusedTables ft;
+    std::string err;
+    if (build_fused_tables(N, TY, ft, err)) return fail(SWE_EINVAL, err);
+    for (int i = 0; i < ft.nseg; i++) {
+        const int4 a = segment_of<TY>(N, i), b = ft.segs[i];
+        if (a.x != b.x || a.y != b.y || a.z != b.z || a.w != b.w)
+            return fail(SWE_EINVAL, "edge segment decode disagrees with the segment table");
+    }
+    // the launch order is a bijection of the tiles, side tiles first
+    const int nt = ft.ntiles, e = side_tiles(ft.ntx, ft.nty);
+    std::vector<char> seen(nt, 0);
+    for (int i = 0; i < nt; i++) {
+        const int t = tile_at(i, ft.ntx, ft.nty);
+        if (t < 0 || t >= nt || seen[t]) return fail(SWE_EINVAL, "tile order is not a bijection");
+        seen[t] = 1;
+        const int r = t / 6, ty = r / ft.ntx, tx = r - ty * ft.ntx;
+        const bool side = tx == 0 || tx == ft.ntx - 1 || ty == 0 || ty == ft.nty - 1;
+        if ((i < e) != side) return fail(SWE_EINVAL, "tile order does not put the side tiles first");
+    }
+    return SWE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int swe_check_tables(int N, int precision)
+{
+    if (N < 4 || N > 4096) return fail(SWE_EINVAL, "swe_check_tables: N out of range");
+    if (precision == SWE_FP32) return check_tables_ty<16>(N);
+    if (precision == SWE_FP64) return check_tables_ty<8>(N);
+    return fail(SWE_EINVAL, "precision must be SWE_FP64 or SWE_FP32");
+}
+
+}  // extern "C"
 #include "swe_ensemble.inc"
 #include "swe_server.inc"

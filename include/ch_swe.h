@@ -151,6 +151,13 @@ int swe_plan_kernel_times(swe_plan *plan, double *stage_ms, int *n_stage, double
 int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *tj, int *tc,
                  int *sp, int *si, int *sj, int *sc, int *sign);
 
+/* Host-only self-check of the fused kernels' plan-time tables for a panel
+ * grid of N and a precision (no counterpart in the reference; runs without a
+ * GPU): the packed cube side table against the topology, the arithmetic
+ * edge-segment decode against the segment table, and the tile launch order
+ * (a bijection, side tiles first).  0 or SWE_EINVAL with the reason. */
+int swe_check_tables(int N, int precision);
+
 /* Ensembles and uneven batches: many independent solves advanced together,
  * every RK4 stage of all members in one launch of the fused stage kernel
  * (a work list of (member, tile) items).  No counterpart in the reference,

@@ -47,3 +47,11 @@ def test_edge_maps_match_oracle(oracle_mod, N, order, kind):
         pytest.skip("sbp42 needs N >= 8")
     o = oracle_mod.Oracle(N, order)
     assert _entries(cs.halo_map(N, order, kind)) == _entries(o.halo_map(kind))
+
+
+@pytest.mark.parametrize("precision", [cs.FP32, cs.FP64])
+def test_fused_tables_selfcheck(precision):
+    """Packed side table, segment decode and tile order, for uniform and
+    edge-aligned tilings (no GPU needed)."""
+    for N in list(range(4, 130)) + [191, 192, 255, 256, 383, 384, 511, 512, 1023]:
+        cs.check_tables(N, precision)
