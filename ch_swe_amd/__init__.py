@@ -41,13 +41,23 @@ EXPORTS = (
     "swe_rk4_steps", "swe_integrals", "swe_halo_map", "swe_ensemble_create",
     "swe_ensemble_destroy", "swe_ensemble_rk4", "swe_ensemble_stream", "swe_ensemble_synchronize",
     "swe_plan_timing", "swe_plan_kernel_times", "swe_server_create", "swe_server_destroy",
-    "swe_server_submit", "swe_server_tick", "swe_server_take", "swe_server_stats",
+    "swe_server_submit", "swe_server_tick", "swe_server_wait", "swe_server_take", "swe_server_stats",
+    "swe_server_counters",
     "swe_check_tables",
 )
 
 
 class SweError(RuntimeError):
-    """A non-zero return code from the C ABI (message from swe_last_error)."""
+    """A non-zero return code from the C ABI (message from swe_last_error);
+    .code holds the SWE_E* value."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+# return codes (include/ch_swe.h)
+EINVAL, EHIP, ENOMEM, ENOTFINITE = -1, -2, -3, -4
 
 
 class _CConfig(ctypes.Structure):
@@ -104,8 +114,10 @@ def lib():
     L.swe_server_submit.argtypes = [_vp, ctypes.POINTER(_CConfig), _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_longlong)]
     L.swe_server_tick.argtypes = [_vp, ctypes.c_int, _ip]
+    L.swe_server_wait.argtypes = [_vp]
     L.swe_server_take.argtypes = [_vp, ctypes.c_longlong, _dp, _dp, _dp, _ip]
     L.swe_server_stats.argtypes = [_vp, _ip, _ip, ctypes.POINTER(ctypes.c_size_t)]
+    L.swe_server_counters.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 3
     _lib = L
     return L
 
@@ -113,7 +125,7 @@ def lib():
 def _check(rc):
     if rc != 0:
         msg = lib().swe_last_error().decode()
-        raise SweError(f"ch_swe error {rc}: {msg}")
+        raise SweError(f"ch_swe error {rc}: {msg}", rc)
 
 
 def _p(a):
@@ -334,7 +346,13 @@ class Server:
         """Queue a solve; returns its id.  kw: Config keywords (physical
         constants)."""
         cfg = Config(N, order, precision, **kw)
-        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, h)]
+        shapes = ((6, N + 1, N), (6, N, N + 1), (6, N + 1, N + 1))
+        arrs = []
+        for name, a, shp in zip("uvh", (u, v, h), shapes):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.size != int(np.prod(shp)):
+                raise ValueError(f"{name}: expected {shp} for N={N}, got {a.shape}")
+            arrs.append(a)
         rid = ctypes.c_longlong()
         _check(lib().swe_server_submit(self.h, ctypes.byref(cfg._c()), *[_p(a) for a in arrs], float(dt),
                                        int(nsteps), ctypes.byref(rid)))
@@ -343,26 +361,44 @@ class Server:
         return rid.value
 
     def tick(self, chunk_steps=25):
-        """Admit, advance up to chunk_steps, retire; returns completions."""
+        """One scheduling round (never waits for the device): retire finished
+        batches, admit, start the next batches; returns completions."""
         n = ctypes.c_int()
         _check(lib().swe_server_tick(self.h, int(chunk_steps), ctypes.byref(n)))
         return n.value
 
+    def wait(self):
+        """Block until the batches in flight have finished."""
+        _check(lib().swe_server_wait(self.h))
+
     def take(self, rid):
-        """(u, v, h) of a finished request, or None if it is still running."""
+        """(u, v, h) of a finished request, or None if it is still running.
+        A request that failed raises SweError (code ENOTFINITE for a
+        non-finite result, whose arrays are then in .result)."""
         N = self._shapes[rid]
         out = [np.empty((6, N + 1, N)), np.empty((6, N, N + 1)), np.empty((6, N + 1, N + 1))]
         ready = ctypes.c_int()
-        _check(lib().swe_server_take(self.h, int(rid), *[_p(a) for a in out], ctypes.byref(ready)))
-        if not ready.value:
+        rc = lib().swe_server_take(self.h, int(rid), *[_p(a) for a in out], ctypes.byref(ready))
+        if rc == 0 and not ready.value:
             return None
         del self._shapes[rid]
+        if rc != 0:
+            err = SweError(f"ch_swe error {rc}: {lib().swe_last_error().decode()}", rc)
+            err.result = tuple(out) if rc == ENOTFINITE else None
+            raise err
         return tuple(out)
 
     def stats(self):
+        """(queued, active, device bytes in use)."""
         q, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
         _check(lib().swe_server_stats(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(b)))
         return q.value, a.value, b.value
+
+    def counters(self):
+        """(requests admitted, cached plans evicted, requests failed)."""
+        x = [ctypes.c_longlong() for _ in range(3)]
+        _check(lib().swe_server_counters(self.h, *[ctypes.byref(v) for v in x]))
+        return tuple(v.value for v in x)
 
 
 def create_timescheme(name):

@@ -385,6 +385,7 @@ struct EngineBase {
     virtual int rk4(void *v0, void *acc, void *ya, void *yb, double dt, int nsteps) = 0;
     virtual bool fused() const = 0;
     virtual size_t elem() const = 0;
+    virtual size_t device_bytes() const = 0;
 };
 
 struct Maps {
@@ -414,11 +415,18 @@ struct Engine : EngineBase {
     swe_plan *pl;
     Dims d;
     int order;
+    GeomParams geo{};
     std::vector<void *> allocs;
-    T *JQ12, *fJ2, *iJa, *Qu, *iJu, *Ju, *Qv, *iJv, *Jv;
+    size_t bytes = 0;  // device bytes allocated so far
+    // per-panel metric, work fields and halo maps of the unfused path, also
+    // used by swe_integrals (co2contra on whole fields): allocated on first use
+    bool full_ready = false;
+    T *JQ12 = nullptr, *fJ2 = nullptr, *iJa = nullptr, *Qu = nullptr, *iJu = nullptr, *Ju = nullptr,
+      *Qv = nullptr, *iJv = nullptr, *Jv = nullptr;
     T *qb, *qt, *ql, *qr;
-    double *wxy, *wu, *wv;
-    T *W1, *W2, *ut, *vt, *Jut, *Jvt, *up, *vp, *div;
+    double *wxy = nullptr, *wu = nullptr, *wv = nullptr;
+    T *W1 = nullptr, *W2 = nullptr, *ut = nullptr, *vt = nullptr, *Jut = nullptr, *Jvt = nullptr,
+      *up = nullptr, *vp = nullptr, *div = nullptr;
     Maps maps;
     double *part = nullptr, *red = nullptr;
     int nblk_red = 512;
@@ -433,18 +441,20 @@ struct Engine : EngineBase {
 
     size_t elem() const override { return sizeof(T); }
     bool fused() const override { return use_fused; }
+    size_t device_bytes() const override { return bytes; }
 
     ~Engine() override
     {
         for (void *p : allocs) (void)hipFree(p);
     }
 
-    int alloc(void **p, size_t bytes)
+    int alloc(void **p, size_t nbytes)
     {
-        hipError_t e = hipMalloc(p, bytes);
+        hipError_t e = hipMalloc(p, nbytes);
         if (e != hipSuccess) return fail(SWE_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
         allocs.push_back(*p);
-        e = hipMemsetAsync(*p, 0, bytes, pl->stream);
+        bytes += nbytes;
+        e = hipMemsetAsync(*p, 0, nbytes, pl->stream);
         if (e != hipSuccess) return fail(SWE_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
         return SWE_OK;
     }
@@ -480,39 +490,51 @@ struct Engine : EngineBase {
         d = p->d;
         order = p->cfg.order;
         int rc = 0;
+        T **qs[] = {&qb, &qt, &ql, &qr};
+        for (T **q : qs)
+            if ((rc = alloc((void **)q, (d.N + 1) * sizeof(T)))) return rc;
+        geo.hx = 0.5 * PI / d.N;
+        geo.radius = p->cfg.radius;
+        geo.omega = p->cfg.omega;
+        for (int k = 0; k < 3; k++) geo.axis[k] = p->cfg.axis[k];
+        geo.coriolis_const = p->cfg.coriolis_const;
+        geo.order = order;
+        k_sync_weights<T><<<blocks_for(d.N + 1), TPB, 0, pl->stream>>>(d.N, geo.hx, qb, qt, ql, qr);
+        HIP_TRY(hipGetLastError());
+        use_fused = !(p->cfg.flags & SWE_FLAG_UNFUSED);
+        nthr = p->cfg.block_threads ? p->cfg.block_threads : F_THREADS;
+        if (use_fused) rc = init_fused(geo);
+        else rc = ensure_full();
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        return SWE_OK;
+    }
+
+    // The unfused path's per-panel metric fields, work fields and halo maps,
+    // and the integral weights and partial sums.  A fused plan allocates them
+    // only when swe_integrals first runs on it.
+    int ensure_full()
+    {
+        if (full_ready) return SWE_OK;
+        int rc = 0;
         T **cf[] = {&JQ12, &fJ2, &iJa, &Qu, &iJu, &Ju, &Qv, &iJv, &Jv,
                     &W1, &W2, &ut, &vt, &Jut, &Jvt, &up, &vp, &div};
         for (T **f : cf)
             if ((rc = field(f))) return rc;
         if ((rc = field(&wxy)) || (rc = field(&wu)) || (rc = field(&wv))) return rc;
-        T **qs[] = {&qb, &qt, &ql, &qr};
-        for (T **q : qs)
-            if ((rc = alloc((void **)q, (d.N + 1) * sizeof(T)))) return rc;
         if ((rc = alloc((void **)&part, 3 * nblk_red * sizeof(double)))) return rc;
         if ((rc = alloc((void **)&red, 4 * sizeof(double)))) return rc;
-
-        GeomParams g;
-        g.hx = 0.5 * PI / d.N;
-        g.radius = p->cfg.radius;
-        g.omega = p->cfg.omega;
-        for (int k = 0; k < 3; k++) g.axis[k] = p->cfg.axis[k];
-        g.coriolis_const = p->cfg.coriolis_const;
-        g.order = order;
         long nxy = 6L * (d.N + 1) * (d.N + 1), nuv = 6L * d.N * (d.N + 1);
-        k_metric<T><<<blocks_for(nxy), TPB, 0, pl->stream>>>(d, g, 0, JQ12, fJ2, iJa, wxy);
-        k_metric<T><<<blocks_for(nuv), TPB, 0, pl->stream>>>(d, g, 1, Qu, iJu, Ju, wu);
-        k_metric<T><<<blocks_for(nuv), TPB, 0, pl->stream>>>(d, g, 2, Qv, iJv, Jv, wv);
-        k_sync_weights<T><<<blocks_for(d.N + 1), TPB, 0, pl->stream>>>(d.N, g.hx, qb, qt, ql, qr);
+        k_metric<T><<<blocks_for(nxy), TPB, 0, pl->stream>>>(d, geo, 0, JQ12, fJ2, iJa, wxy);
+        k_metric<T><<<blocks_for(nuv), TPB, 0, pl->stream>>>(d, geo, 1, Qu, iJu, Ju, wu);
+        k_metric<T><<<blocks_for(nuv), TPB, 0, pl->stream>>>(d, geo, 2, Qv, iJv, Jv, wv);
         HIP_TRY(hipGetLastError());
-
         // maps: field slots 0 and 1 are the (u-type, v-type) components
         if ((rc = upload_map(0, 1, {0, 0}, &maps.xy, &maps.n_xy))) return rc;
         if ((rc = upload_map(1, order == 21 ? 2 : 3, {0, 1}, &maps.ch, &maps.n_ch))) return rc;
         if ((rc = upload_map(2, 1, {0, 1}, &maps.xyv, &maps.n_xyv))) return rc;
-        use_fused = !(p->cfg.flags & SWE_FLAG_UNFUSED);
-        nthr = p->cfg.block_threads ? p->cfg.block_threads : F_THREADS;
-        if ((rc = init_fused(g))) return rc;
         HIP_TRY(hipStreamSynchronize(pl->stream));
+        full_ready = true;
         return SWE_OK;
     }
 
@@ -684,6 +706,8 @@ struct Engine : EngineBase {
 
     int integrals(void *st, double *mass, double *energy) override
     {
+        int rc = ensure_full();
+        if (rc) return rc;
         const int N = d.N;
         T *u = (T *)st, *v = u + d.F, *h = u + 2 * d.F;
         long nxy = 6L * (N + 1) * (N + 1), nuv = 6L * N * (N + 1);
@@ -714,6 +738,30 @@ int check_state(swe_plan *plan, swe_state *s, const char *what)
 {
     if (!s) return fail(SWE_EINVAL, std::string(what) + " is NULL");
     if (s->plan != plan) return fail(SWE_EINVAL, std::string(what) + " belongs to another plan");
+    return SWE_OK;
+}
+
+// The checks swe_plan_create applies to a configuration (also run by
+// swe_server_submit, so that a request that could never be admitted is
+// refused when it is submitted).
+int validate_config(const swe_config *cfg, const char *who)
+{
+    const std::string w = std::string(who) + ": ";
+    if (!cfg) return fail(SWE_EINVAL, w + "NULL configuration");
+    if (cfg->order != 21 && cfg->order != 42) return fail(SWE_EINVAL, w + "order must be 21 or 42");
+    if (cfg->order == 21 && cfg->N < 4) return fail(SWE_EINVAL, w + "sbp21 needs N >= 4");
+    if (cfg->order == 42 && cfg->N < 8) return fail(SWE_EINVAL, w + "sbp42 needs N >= 8");
+    if (cfg->N > 4096) return fail(SWE_EINVAL, w + "N above 4096 is not supported");
+    if (cfg->precision != SWE_FP64 && cfg->precision != SWE_FP32)
+        return fail(SWE_EINVAL, w + "precision must be SWE_FP64 or SWE_FP32");
+    if (cfg->block_threads != 0 && cfg->block_threads != F_THREADS)
+        return fail(SWE_EINVAL, w + "block_threads must be 0 or 512 (the fused kernels' workgroup size)");
+    if (cfg->flags & ~(SWE_FLAG_UNFUSED | SWE_FLAG_LINEAR_BLOCKS | SWE_FLAG_GENERAL_TILES))
+        return fail(SWE_EINVAL, w + "unknown bits in flags");
+    if (!(cfg->radius > 0) || !std::isfinite(cfg->radius) || !std::isfinite(cfg->grav) ||
+        !std::isfinite(cfg->h_mean) || !std::isfinite(cfg->omega) || !std::isfinite(cfg->axis[0]) ||
+        !std::isfinite(cfg->axis[1]) || !std::isfinite(cfg->axis[2]))
+        return fail(SWE_EINVAL, w + "non-finite or non-positive physical constant");
     return SWE_OK;
 }
 
@@ -778,17 +826,8 @@ int swe_plan_create(const swe_config *cfg, swe_plan **out)
 {
     if (!cfg || !out) return fail(SWE_EINVAL, "swe_plan_create: NULL argument");
     *out = nullptr;
-    if (cfg->order != 21 && cfg->order != 42) return fail(SWE_EINVAL, "order must be 21 or 42");
-    if (cfg->order == 21 && cfg->N < 4) return fail(SWE_EINVAL, "sbp21 needs N >= 4");
-    if (cfg->order == 42 && cfg->N < 8) return fail(SWE_EINVAL, "sbp42 needs N >= 8");
-    if (cfg->N > 4096) return fail(SWE_EINVAL, "N above 4096 is not supported");
-    if (cfg->precision != SWE_FP64 && cfg->precision != SWE_FP32)
-        return fail(SWE_EINVAL, "precision must be SWE_FP64 or SWE_FP32");
-    if (cfg->block_threads != 0 && cfg->block_threads != F_THREADS)
-        return fail(SWE_EINVAL, "block_threads must be 0 or 512 (the fused kernels' workgroup size)");
-    if (!(cfg->radius > 0) || !std::isfinite(cfg->grav) || !std::isfinite(cfg->h_mean) ||
-        !std::isfinite(cfg->omega))
-        return fail(SWE_EINVAL, "non-finite or non-positive physical constant");
+    int rc0 = validate_config(cfg, "swe_plan_create");
+    if (rc0) return rc0;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SWE_EHIP, "no HIP device available");
     auto *p = new swe_plan();
@@ -809,7 +848,6 @@ int swe_plan_create(const swe_config *cfg, swe_plan **out)
         p->eng.reset(en);
         rc = en->init(p);
     }
-    for (int k = 0; k < 5 && rc == SWE_OK; k++) rc = swe_state_create(p, &p->rk[k]);
     if (rc != SWE_OK) {
         std::string msg = g_err;
         swe_plan_destroy(p);
@@ -922,6 +960,12 @@ int swe_rk4_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps)
     if (nsteps < 0) return fail(SWE_EINVAL, "nsteps < 0");
     if (!std::isfinite(dt)) return fail(SWE_EINVAL, "dt is not finite");
     if (nsteps == 0) return SWE_OK;
+    // the RK4 registers are made on first use: the fused path keeps three
+    // (the increment accumulator and two stage states), the unfused one the
+    // reference's five (k1..k4, y)
+    const int nreg = plan->eng->fused() ? 3 : 5;
+    for (int k = 0; k < nreg; k++)
+        if (!plan->rk[k] && (rc = swe_state_create(plan, &plan->rk[k]))) return rc;
     swe_state *k1 = plan->rk[0], *k2 = plan->rk[1], *k3 = plan->rk[2], *k4 = plan->rk[3], *y = plan->rk[4];
     if (plan->eng->fused()) return plan->eng->rk4(v0->buf, k1->buf, k2->buf, k3->buf, dt, nsteps);
     for (int it = 0; it < nsteps; it++) {

@@ -255,8 +255,12 @@ def analyze(name, body, n_entry):
 
 
 def check_file(path, pat=r'.'):
-    """{kernel: [(line, mnemonic, operands, sgpr, block)]} for kernels matching pat."""
+    """{kernel: [(line, mnemonic, operands, sgpr, block)]} for kernels matching
+    pat.  An out-of-line device function (not a kernel) is reported as one
+    finding: the analysis assumes a kernel's entry state, and the fused
+    kernels are meant to inline everything they call."""
     lines = open(path).read().split('\n')
+    kernels = set(re.findall(r'^\s*\.amdhsa_kernel\s+(\S+)', '\n'.join(lines), re.M))
     kern = []
     for i, l in enumerate(lines):
         m = re.match(r'^(_Z\S+|k_\S+):\s*;\s*@', l)
@@ -265,6 +269,9 @@ def check_file(path, pat=r'.'):
     found = {}
     for name, start in kern:
         if not re.search(pat, name):
+            continue
+        if kernels and name not in kernels:
+            found[name] = [(start + 1, 'out-of-line device function', [], -1, 'entry')]
             continue
         end = start
         while end < len(lines) and not lines[end].startswith('.Lfunc_end'):

@@ -196,14 +196,15 @@ def sh_state(N, order=42, lmax=32, seed=0, rms=20.0, grav=9.81, omega=7.292e-5, 
     return out["u"], out["v"], out["h"]
 
 
-def ensemble_states(N, order, members, lmax=16, seed=0, basis=8, rms=20.0, radius=6.371e6):
+def ensemble_states(N, order, members, lmax=16, seed=0, basis=8, rms=20.0, radius=6.371e6, workers=None):
     """Seeded ensemble initial states: member m gets a degree-lmax
     spherical-harmonic field with coefficients ~ N(0,1) l^-2 (a unit-norm
     random combination of `basis` sh_state fields, which has the same
     coefficient distribution) with its balanced winds, plus a Gaussian bump of
     amplitude U(1, 20) m, width U(2e5, 1e6) m and a centre uniform on the
-    sphere (no wind).  Returns a list of (u, v, h)."""
-    base = [sh_state(N, order, lmax=lmax, seed=seed * 1000 + k, rms=rms) for k in range(basis)]
+    sphere (no wind).  Returns a list of (u, v, h).  workers: as for
+    sh_state (1 in a process that has already used the GPU)."""
+    base = [sh_state(N, order, lmax=lmax, seed=seed * 1000 + k, rms=rms, workers=workers) for k in range(basis)]
     rng = np.random.default_rng(seed)
     w = rng.standard_normal((members, basis))
     w /= np.linalg.norm(w, axis=1, keepdims=True)

@@ -64,7 +64,7 @@ typedef struct {
     double omega;       /* rotation rate [1/s] */
     double axis[3];     /* rotation axis */
     int flags;          /* SWE_FLAG_* bits */
-    int block_threads;  /* workgroup size of the fused kernels: 0 (default 512), 256 or 512 */
+    int block_threads;  /* workgroup size of the fused kernels: 0 or 512 (both mean 512) */
 } swe_config;
 
 const char *swe_last_error(void);
@@ -73,7 +73,8 @@ const char *swe_version(void);
 /* Plan: geometry (FP64 metric built on the device), edge maps, work fields,
  * stream.  Replaces create_domain + create_linear_swm_operator +
  * create_timescheme("rk4") (models/shallow_water/shallow_water_mod.f90
- * L70-93). */
+ * L70-93).  The RK4 registers are allocated by the first swe_rk4_steps call,
+ * the fields swe_integrals needs by its first call. */
 int swe_plan_create(const swe_config *cfg, swe_plan **out);
 int swe_plan_destroy(swe_plan *plan);
 int swe_plan_npts(const swe_plan *plan, int *n_u, int *n_v, int *n_h);
@@ -112,24 +113,38 @@ int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy);
 /* Serving: continuous batching of independent solves of mixed size and
  * precision under a device-memory budget (no counterpart in the reference,
  * which runs one solve per program).  Requests are queued with a copy of
- * their initial state; each tick admits queued requests in arrival order
- * while their device memory fits the budget, advances every active request by
- * up to chunk_steps RK4 steps (all active requests of one precision together,
- * one launch per stage and SBP order, the two precisions on their own
- * streams), and retires the finished ones.  Every request computes exactly
- * what swe_rk4_steps computes for it alone.  Single-threaded use. */
+ * their initial state.  Each precision has a lane with at most one batch of
+ * RK4 steps in flight (all its requests together, one launch per stage and
+ * SBP order; the two lanes on their own streams, so their work overlaps).
+ * Every request computes exactly what swe_rk4_steps computes for it alone.
+ * Single-threaded use. */
 typedef struct swe_server swe_server;
 int swe_server_create(size_t budget_bytes, swe_server **out);
 int swe_server_destroy(swe_server *srv);
-/* cfg as for swe_plan_create (fused path); u, v, h packed FP64 (copied). */
+/* cfg is checked as swe_plan_create checks it (fused path only); u, v, h
+ * packed FP64 of the sizes of cfg->N (copied). */
 int swe_server_submit(swe_server *srv, const swe_config *cfg, const double *u, const double *v,
                       const double *h, double dt, int nsteps, long long *id);
-/* Admit, advance, retire; *completed = requests finished in this call. */
+/* One scheduling round; never waits for the device.  Retires the requests
+ * whose batch has finished, admits queued requests in arrival order while
+ * their device memory fits the budget (releasing idle cached plans first),
+ * and starts the next batch on each idle lane: the lane's largest request
+ * advances chunk_steps steps, a request with fewer cells proportionally more.
+ * *completed = requests retired in this call.  A request that cannot be
+ * admitted is retired with the error instead of blocking the queue. */
 int swe_server_tick(swe_server *srv, int chunk_steps, int *completed);
-/* *ready = 1 and the final (u, v, h) copied out (and the request forgotten)
- * once request id has finished; *ready = 0 before. */
+/* Block until the batches in flight have finished (the next tick retires). */
+int swe_server_wait(swe_server *srv);
+/* *ready = 0 while request id runs.  Once it has finished: *ready = 1, the
+ * request is forgotten, and the return value is its status: SWE_OK with the
+ * final (u, v, h) copied out; SWE_ENOTFINITE with the (non-finite) result
+ * copied out; or the error that kept it from being admitted (nothing copied).
+ * swe_last_error() names the request. */
 int swe_server_take(swe_server *srv, long long id, double *u, double *v, double *h, int *ready);
 int swe_server_stats(swe_server *srv, int *queued, int *active, size_t *bytes_in_use);
+/* Totals since creation: requests admitted, cached plans released to make
+ * room, requests retired with an error status. */
+int swe_server_counters(swe_server *srv, long long *admitted, long long *evicted_plans, long long *failed);
 
 /* Measurement aid (no counterpart in the reference): with timing enabled,
  * every launch of the fused stage kernel and of the edge-record kernel on

@@ -1,12 +1,14 @@
 """GPU: the serving loop (swe_server_*).  Requests of mixed size, order and
-precision, admitted at different ticks under a memory budget, each give
-exactly what the same solve gives alone."""
+precision, admitted at different times under a memory budget, each give
+exactly what the same solve gives alone; a request that fails is reported
+on its own while the others finish."""
 import sys
 
 import numpy as np
 import pytest
 
 import ch_swe_amd as cs
+from ch_swe_amd import testcases as tc
 
 pytestmark = pytest.mark.gpu
 
@@ -31,6 +33,24 @@ def _alone(N, order, precision, s0, dt, nsteps):
     st = op.create_stvec().upload(*s0)
     cs.create_timescheme("rk4").step(st, op, dt, nsteps)
     return st.download()
+
+
+def _drain(srv, ids, results, limit=2000):
+    """Tick and wait until every id in ids has a result (or an error)."""
+    for _ in range(limit):
+        srv.tick(7)
+        for k, rid in ids.items():
+            if k not in results:
+                try:
+                    res = srv.take(rid)
+                except cs.SweError as e:
+                    res = e
+                if res is not None:
+                    results[k] = res
+        if len(results) == len(ids):
+            return
+        srv.wait()
+    raise AssertionError("requests did not finish")
 
 
 # (N, order, precision, dt, nsteps, tick at which it is submitted)
@@ -58,6 +78,7 @@ def test_mixed_requests_match_alone(oracle_mod, budget):
                 res = srv.take(rid)
                 if res is not None:
                     results[k] = res
+        srv.wait()
         tick += 1
         assert tick < 500
     for k, (N, order, prec, dt, ns, _) in enumerate(REQS):
@@ -72,6 +93,78 @@ def test_mixed_requests_match_alone(oracle_mod, budget):
     assert srv.stats()[:2] == (0, 0)
 
 
+def _footprint(N, order, prec):
+    """(plan bytes, request bytes) the server charges, read from its stats."""
+    srv = cs.Server(1 << 36)
+    rid = srv.submit(*_state(N, 1), 100.0, 1, N, order, prec)
+    srv.tick(1)
+    both = srv.stats()[2]
+    srv.wait()
+    while srv.take(rid) is None:
+        srv.tick(1)
+        srv.wait()
+    plan = srv.stats()[2]
+    srv.close()
+    return plan, both - plan
+
+
+def test_evicting_an_idle_plan_ahead_of_a_reused_one():
+    """Plan A is cached idle and listed before plan B; a request reusing B
+    arrives while another B request runs, under a budget that fits only
+    after A is released.  The reused plan must stay the one the request is
+    bound to: both B results equal their solo runs bitwise."""
+    _say("server eviction ahead of reuse")
+    A, B = (24, 42, cs.FP64), (12, 21, cs.FP64)
+    plan_a, req_a = _footprint(*A)
+    plan_b, req_b = _footprint(*B)
+    # A, B and one B request fit; a second B request fits only once A is gone
+    budget = plan_a + plan_b + req_b + req_b // 2
+    assert plan_a + req_a <= budget and plan_b + 2 * req_b <= budget
+    srv = cs.Server(budget)
+    sa = _state(A[0], 7)
+    ra = srv.submit(*sa, 500.0, 3, *A)
+    out = {}
+    _drain(srv, {"a": ra}, out)
+    assert srv.counters()[1] == 0
+    s1, s2 = _state(B[0], 8), _state(B[0], 9)
+    r1 = srv.submit(*s1, 700.0, 200, *B)
+    srv.tick(7)  # B1 admitted next to the idle plan A
+    assert srv.stats()[1] == 1
+    r2 = srv.submit(*s2, 600.0, 30, *B)
+    _drain(srv, {"b1": r1, "b2": r2}, out)
+    assert srv.counters()[1] == 1, "plan A was not evicted"
+    for key, s, dt, ns in (("b1", s1, 700.0, 200), ("b2", s2, 600.0, 30)):
+        for g, r in zip(out[key], _alone(B[0], B[1], B[2], s, dt, ns)):
+            assert _rel(g, r) == 0.0, key
+    for g, r in zip(out["a"], _alone(A[0], A[1], A[2], sa, 500.0, 3)):
+        assert _rel(g, r) == 0.0
+
+
+def test_nonfinite_request_is_isolated():
+    """One request with dt far beyond the CFL limit blows up; it is retired
+    with SWE_ENOTFINITE while the requests batched with it finish bitwise
+    equal to their solo runs."""
+    _say("server non-finite isolation")
+    srv = cs.Server(1 << 34)
+    N, order = 24, 42
+    dt_ok = tc.cfl_dt(N, 0.5)
+    jobs = {"ok1": (cs.FP32, dt_ok, 60, 11), "bad": (cs.FP32, 60 * dt_ok, 60, 12),
+            "ok2": (cs.FP32, 0.7 * dt_ok, 45, 13), "ok64": (cs.FP64, dt_ok, 50, 14)}
+    s0 = {k: _state(N, seed) for k, (_, _, _, seed) in jobs.items()}
+    ids = {k: srv.submit(*s0[k], dt, ns, N, order, prec) for k, (prec, dt, ns, _) in jobs.items()}
+    out = {}
+    _drain(srv, ids, out)
+    err = out["bad"]
+    assert isinstance(err, cs.SweError) and err.code == cs.ENOTFINITE
+    assert not all(np.isfinite(a).all() for a in err.result)
+    for k in ("ok1", "ok2", "ok64"):
+        prec, dt, ns, _ = jobs[k]
+        assert not isinstance(out[k], Exception), k
+        for g, r in zip(out[k], _alone(N, order, prec, s0[k], dt, ns)):
+            assert _rel(g, r) == 0.0, k
+    assert srv.counters()[2] == 1
+
+
 def test_server_rejects_bad_requests():
     _say("server errors")
     srv = cs.Server(1 << 20)
@@ -81,8 +174,11 @@ def test_server_rejects_bad_requests():
     big = cs.Server(1 << 32)
     with pytest.raises(cs.SweError, match="order"):
         big.submit(*s, 600.0, 3, 16, 63, cs.FP64)
+    with pytest.raises(cs.SweError, match="block_threads"):
+        big.submit(*s, 600.0, 3, 16, 42, cs.FP64, block_threads=256)
+    with pytest.raises(ValueError, match="expected"):
+        big.submit(s[0][:, :-1], s[1], s[2], 600.0, 3, 16, 42, cs.FP64)
     with pytest.raises(cs.SweError, match="unknown request"):
-        cs.lib()
         import ctypes
 
         r = ctypes.c_int()

@@ -140,3 +140,20 @@ def test_execnz_on_a_mask_still_zero_is_not_a_path(tmp_path):
     # a mask that is not known to be zero on the skip path keeps the finding
     live = EXECNZ.replace("\ts_mov_b64 s[20:21], 0\n", "\ts_mov_b64 s[20:21], exec\n")
     assert {r[3] for r in _check(live, tmp_path)["_Z4kernv"]} == {18, 19}
+
+
+def test_out_of_line_device_function_is_refused(tmp_path):
+    """A device function the compiler did not inline is reported: the
+    analysis assumes a kernel's entry state."""
+    text = GOOD + """
+_Z6helperPf: ; @_Z6helperPf
+; %bb.0:
+\ts_waitcnt vmcnt(0)
+\ts_setpc_b64 s[30:31]
+.Lfunc_end1:
+\t.amdhsa_kernel _Z4kernv
+\t.end_amdhsa_kernel
+"""
+    found = _check(text, tmp_path)
+    assert list(found) == ["_Z6helperPf"]
+    assert found["_Z6helperPf"][0][1] == "out-of-line device function"
