@@ -47,7 +47,7 @@ def _drain(srv, ids, results, limit=2000):
                     res = e
                 if res is not None:
                     results[k] = res
-        if len(results) == len(ids):
+        if all(k in results for k in ids):
             return
         srv.wait()
     raise AssertionError("requests did not finish")
