@@ -69,6 +69,21 @@ __device__ inline double sbp_weight(int idx, int n, bool centre, int order)
     return w;
 }
 
+// SBP quadrature weight J hx hy Ax(i) Ay(j) of point (i, j) on mesh m (0 xy,
+// 1 u at x centres, 2 v at y centres), FP64 (quadrature_factory_mod.f90
+// L89-188: create_C_sbp_quadrature)
+__device__ inline double quad_weight(int N, int order, double hx, int m, int i, int j)
+{
+    const int xt = m == 1, yt = m == 2;
+    const int nx = xt ? N : N + 1, ny = yt ? N : N + 1;
+    const double al = -0.25 * PI + (i - 1 + 0.5 * xt) * hx;
+    const double be = -0.25 * PI + (j - 1 + 0.5 * yt) * hx;
+    const double X = tan(al), Y = tan(be);
+    const double s2 = 1.0 + X * X + Y * Y, s = sqrt(s2);
+    const double J = (1.0 + X * X) * (1.0 + Y * Y) / (s2 * s);
+    return J * hx * hx * sbp_weight(i, nx, xt, order) * sbp_weight(j, ny, yt, order);
+}
+
 // mesh m: 0 xy, 1 u (x centres), 2 v (y centres).  Writes FP64 J, Qi, f.
 template <typename T>
 __global__ void k_metric(Dims d, GeomParams g, int m, T *c0, T *c1, T *c2, double *w)
@@ -88,7 +103,7 @@ __global__ void k_metric(Dims d, GeomParams g, int m, T *c0, T *c1, T *c2, doubl
     double q11 = s2 / (1.0 + X * X), q12 = s2 * X * Y / ((1.0 + X * X) * (1.0 + Y * Y)),
            q22 = s2 / (1.0 + Y * Y);
     long o = off(d, p, i, j);
-    w[o] = J * g.hx * g.hx * sbp_weight(i, nx, xt, g.order) * sbp_weight(j, ny, yt, g.order);
+    w[o] = quad_weight(N, g.order, g.hx, m, i, j);
     if (m == 0) {
         // r = ex X/s + ey Y/s - n 1/s (ecs_proto2realface with identity rotation)
         double r[3];

@@ -381,7 +381,9 @@ struct EngineBase {
     virtual ~EngineBase() = default;
     virtual int apply(void *out, void *in) = 0;
     virtual int axpy(void *out, double s1, void *x, double s2, void *y, int update) = 0;
-    virtual int integrals(void *st, double *mass, double *energy) = 0;
+    // part: >= 3 INT_MAX_BLOCKS doubles of partial sums, red: 4 doubles (device)
+    virtual int integrals(void *st, double *part, double *red, double *mass, double *energy) = 0;
+    virtual int integral_buffers(double **part, double **red) = 0;
     virtual int rk4(void *v0, void *acc, void *ya, void *yb, double dt, int nsteps) = 0;
     virtual bool fused() const = 0;
     virtual size_t elem() const = 0;
@@ -398,6 +400,7 @@ struct Maps {
 struct swe_plan {
     swe_config cfg;
     Dims d;
+    int device = 0;  // the HIP device current when the plan was made
     hipStream_t stream = nullptr;
     std::unique_ptr<EngineBase> eng;
     swe_state *rk[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // k1..k4, y
@@ -418,8 +421,8 @@ struct Engine : EngineBase {
     GeomParams geo{};
     std::vector<void *> allocs;
     size_t bytes = 0;  // device bytes allocated so far
-    // per-panel metric, work fields and halo maps of the unfused path, also
-    // used by swe_integrals (co2contra on whole fields): allocated on first use
+    // per-panel metric, work fields and halo maps of the unfused path (made
+    // by unfused plans only)
     bool full_ready = false;
     T *JQ12 = nullptr, *fJ2 = nullptr, *iJa = nullptr, *Qu = nullptr, *iJu = nullptr, *Ju = nullptr,
       *Qv = nullptr, *iJv = nullptr, *Jv = nullptr;
@@ -429,7 +432,8 @@ struct Engine : EngineBase {
       *up = nullptr, *vp = nullptr, *div = nullptr;
     Maps maps;
     double *part = nullptr, *red = nullptr;
-    int nblk_red = 512;
+    static constexpr int nblk_red = 512;  // grid of the unfused path's k_reduce
+    static_assert(nblk_red <= INT_MAX_BLOCKS, "partial sums of k_reduce must fit the integral buffers");
 
     // fused path
     static constexpr int TY = sizeof(T) == 4 ? 16 : 8;
@@ -503,6 +507,9 @@ struct Engine : EngineBase {
         HIP_TRY(hipGetLastError());
         use_fused = !(p->cfg.flags & SWE_FLAG_UNFUSED);
         nthr = p->cfg.block_threads ? p->cfg.block_threads : F_THREADS;
+        if ((rc = alloc((void **)&part, 3 * INT_MAX_BLOCKS * sizeof(double))) ||
+            (rc = alloc((void **)&red, 4 * sizeof(double))))
+            return rc;
         if (use_fused) rc = init_fused(geo);
         else rc = ensure_full();
         if (rc) return rc;
@@ -511,8 +518,7 @@ struct Engine : EngineBase {
     }
 
     // The unfused path's per-panel metric fields, work fields and halo maps,
-    // and the integral weights and partial sums.  A fused plan allocates them
-    // only when swe_integrals first runs on it.
+    // and its integral weights.
     int ensure_full()
     {
         if (full_ready) return SWE_OK;
@@ -522,8 +528,6 @@ struct Engine : EngineBase {
         for (T **f : cf)
             if ((rc = field(f))) return rc;
         if ((rc = field(&wxy)) || (rc = field(&wu)) || (rc = field(&wv))) return rc;
-        if ((rc = alloc((void **)&part, 3 * nblk_red * sizeof(double)))) return rc;
-        if ((rc = alloc((void **)&red, 4 * sizeof(double)))) return rc;
         long nxy = 6L * (d.N + 1) * (d.N + 1), nuv = 6L * d.N * (d.N + 1);
         k_metric<T><<<blocks_for(nxy), TPB, 0, pl->stream>>>(d, geo, 0, JQ12, fJ2, iJa, wxy);
         k_metric<T><<<blocks_for(nuv), TPB, 0, pl->stream>>>(d, geo, 1, Qu, iJu, Ju, wu);
@@ -535,6 +539,14 @@ struct Engine : EngineBase {
         if ((rc = upload_map(2, 1, {0, 1}, &maps.xyv, &maps.n_xyv))) return rc;
         HIP_TRY(hipStreamSynchronize(pl->stream));
         full_ready = true;
+        return SWE_OK;
+    }
+
+    // the plan's buffers for the partial sums and totals of swe_integrals
+    int integral_buffers(double **pp, double **pr) override
+    {
+        *pp = part;
+        *pr = red;
         return SWE_OK;
     }
 
@@ -704,28 +716,45 @@ struct Engine : EngineBase {
         return SWE_OK;
     }
 
-    int integrals(void *st, double *mass, double *energy) override
+    template <int O>
+    void fused_integrals(const T *st, double *prt, double *rd)
     {
-        int rc = ensure_full();
-        if (rc) return rc;
+        const int grid = std::min(ntiles, INT_MAX_BLOCKS);
+        k_integrals<T, O, TY><<<grid, F_THREADS, 0, pl->stream>>>(d, cf, st, ntx, nty, fwx, fwy, prt);
+        k_reduce_final<<<1, 64, 0, pl->stream>>>(prt, grid, rd);
+    }
+
+    // FP64 integrals of state st; prt / rd: device buffers for the partial
+    // sums (3 INT_MAX_BLOCKS doubles) and the totals (4 doubles).  The fused
+    // path computes ut, vt tile by tile (no per-panel fields); the unfused
+    // path runs its own co2contra kernels into its work fields.
+    int integrals(void *st, double *prt, double *rd, double *mass, double *energy) override
+    {
         const int N = d.N;
         T *u = (T *)st, *v = u + d.F, *h = u + 2 * d.F;
-        long nxy = 6L * (N + 1) * (N + 1), nuv = 6L * N * (N + 1);
         hipStream_t s = pl->stream;
-        if (order == 21) {
-            k_xy_w<T, 21><<<blocks_for(nxy), TPB, 0, s>>>(d, u, v, JQ12, W1, W2);
-            k_contra<T, 21><<<blocks_for(2 * nuv), TPB, 0, s>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut, vt,
-                                                                Jut, Jvt);
+        if (use_fused) {
+            if (order == 21) fused_integrals<21>(u, prt, rd);
+            else fused_integrals<42>(u, prt, rd);
         } else {
-            k_xy_w<T, 42><<<blocks_for(nxy), TPB, 0, s>>>(d, u, v, JQ12, W1, W2);
-            k_contra<T, 42><<<blocks_for(2 * nuv), TPB, 0, s>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut, vt,
-                                                                Jut, Jvt);
+            int rc = ensure_full();
+            if (rc) return rc;
+            long nxy = 6L * (N + 1) * (N + 1), nuv = 6L * N * (N + 1);
+            if (order == 21) {
+                k_xy_w<T, 21><<<blocks_for(nxy), TPB, 0, s>>>(d, u, v, JQ12, W1, W2);
+                k_contra<T, 21><<<blocks_for(2 * nuv), TPB, 0, s>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut,
+                                                                    vt, Jut, Jvt);
+            } else {
+                k_xy_w<T, 42><<<blocks_for(nxy), TPB, 0, s>>>(d, u, v, JQ12, W1, W2);
+                k_contra<T, 42><<<blocks_for(2 * nuv), TPB, 0, s>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut,
+                                                                    vt, Jut, Jvt);
+            }
+            k_reduce<T><<<nblk_red, TPB, 0, s>>>(d, h, u, v, ut, vt, wxy, wu, wv, prt);
+            k_reduce_final<<<1, 64, 0, s>>>(prt, nblk_red, rd);
         }
-        k_reduce<T><<<nblk_red, TPB, 0, s>>>(d, h, u, v, ut, vt, wxy, wu, wv, part);
-        k_reduce_final<<<1, 64, 0, s>>>(part, nblk_red, red);
         HIP_TRY(hipGetLastError());
         double r[3];
-        HIP_TRY(hipMemcpyAsync(r, red, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(r, rd, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         const swe_config &c = pl->cfg;
         if (mass) *mass = r[0];
@@ -833,7 +862,8 @@ int swe_plan_create(const swe_config *cfg, swe_plan **out)
     auto *p = new swe_plan();
     p->cfg = *cfg;
     p->d = make_dims(cfg->N);
-    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    hipError_t e = hipGetDevice(&p->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete p;
         return fail(SWE_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -1012,7 +1042,9 @@ int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy)
     if (!plan) return fail(SWE_EINVAL, "NULL plan");
     int rc = check_state(plan, st, "state");
     if (rc) return rc;
-    return plan->eng->integrals(st->buf, mass, energy);
+    double *part = nullptr, *red = nullptr;
+    if ((rc = plan->eng->integral_buffers(&part, &red))) return rc;
+    return plan->eng->integrals(st->buf, part, red, mass, energy);
 }
 
 int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *tj, int *tc, int *sp, int *si,

@@ -73,8 +73,7 @@ const char *swe_version(void);
 /* Plan: geometry (FP64 metric built on the device), edge maps, work fields,
  * stream.  Replaces create_domain + create_linear_swm_operator +
  * create_timescheme("rk4") (models/shallow_water/shallow_water_mod.f90
- * L70-93).  The RK4 registers are allocated by the first swe_rk4_steps call,
- * the fields swe_integrals needs by its first call. */
+ * L70-93).  The RK4 registers are allocated by the first swe_rk4_steps call. */
 int swe_plan_create(const swe_config *cfg, swe_plan **out);
 int swe_plan_destroy(swe_plan *plan);
 int swe_plan_npts(const swe_plan *plan, int *n_u, int *n_v, int *n_h);

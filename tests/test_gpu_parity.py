@@ -168,16 +168,29 @@ def test_stvec_assign_update(oracle_mod):
         assert np.abs(g - 4 * y).max() < 1e-15
 
 
-@pytest.mark.parametrize("order", [21, 42])
-def test_integrals_match_oracle(oracle_mod, order):
-    N = 20
+# (order, N, precision, flags): the fused tile-wise integrals over single-tile
+# panels, uneven last tiles and more tiles than the kernel has blocks (N=97
+# FP64: 4 x 13 tiles per panel); the unfused path; FP32 states (ut, vt in
+# FP32, the sums in FP64: relative bar 1e-6)
+INTEGRAL_CASES = [(21, 5, cs.FP64, False), (21, 20, cs.FP64, False), (21, 48, cs.FP64, False),
+                  (42, 8, cs.FP64, False), (42, 20, cs.FP64, False), (42, 57, cs.FP64, False),
+                  (42, 97, cs.FP64, False), (42, 20, cs.FP64, True), (21, 33, cs.FP64, True),
+                  (42, 40, cs.FP32, False), (21, 31, cs.FP32, False)]
+
+
+@pytest.mark.parametrize("order,N,precision,unfused", INTEGRAL_CASES)
+def test_integrals_match_oracle(oracle_mod, order, N, precision, unfused):
+    _say(f"integrals order={order} N={N} precision={precision} unfused={unfused}")
     o = oracle_mod.Oracle(N, order)
-    u, v, h = _random_state(o, 21)
-    op = _op(N, order)
+    u, v, h = _random_state(o, 21 + N)
+    op = _op(N, order, precision, unfused=unfused)
     st = op.create_stvec().upload(u, v, h)
     m, e = op.integrals(st)
-    assert abs(m - o.mass(h)) < 1e-12 * np.abs(h).max() * 4 * np.pi
-    assert abs(e / o.energy(u, v, h) - 1) < 1e-12
+    bar = 1e-12 if precision == cs.FP64 else 1e-6
+    assert abs(m - o.mass(h)) < bar * np.abs(h).max() * 4 * np.pi
+    assert abs(e / o.energy(u, v, h) - 1) < bar
+    # a second call on the same state gives the same numbers
+    assert op.integrals(st) == (m, e)
 
 
 def test_errors_are_reported():
