@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1045,6 +1046,55 @@ int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy)
     double *part = nullptr, *red = nullptr;
     if ((rc = plan->eng->integral_buffers(&part, &red))) return rc;
     return plan->eng->integrals(st->buf, part, red, mass, energy);
+}
+
+}  // extern "C"
+
+// ---- swe_integrals_noalloc: the partial sums in a static device array ----------
+// The array holds the partial sums of one swe_integrals call (3 per block of
+// the integral kernels, at most INT_MAX_BLOCKS blocks) and its 3 totals:
+// 24.6 KB, independent of N, on every device the library runs on until the
+// process ends.  Every block writes its partial sums before the final sum
+// reads them, so nothing needs clearing between calls.  Calls that overlap in
+// time on one device share the array and race.
+static __device__ __align__(256) double scratch[3 * INT_MAX_BLOCKS + 4];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[SCRATCH_MAX_DEVICES] = {};  // device address of scratch, per device
+
+// the device address of scratch on device dev (looked up once per device,
+// with that device current)
+int scratch_on(int dev, double **out)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail(SWE_EINVAL, "swe_integrals_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        int prev = 0;
+        HIP_TRY(hipGetDevice(&prev));
+        if (prev != dev) HIP_TRY(hipSetDevice(dev));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (prev != dev) HIP_TRY(hipSetDevice(prev));
+        if (e != hipSuccess) return fail(SWE_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = (double *)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return SWE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int swe_integrals_noalloc(swe_plan *plan, swe_state *st, double *mass, double *energy)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    int rc = check_state(plan, st, "state");
+    if (rc) return rc;
+    double *buf = nullptr;
+    if ((rc = scratch_on(plan->device, &buf))) return rc;
+    return plan->eng->integrals(st->buf, buf, buf + 3 * INT_MAX_BLOCKS, mass, energy);
 }
 
 int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *tj, int *tc, int *sp, int *si,

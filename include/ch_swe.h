@@ -109,6 +109,15 @@ int swe_rk4_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps);
  * energy = g/2 <h,h> + H0/2 (<u,ut> + <v,vt>). */
 int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy);
 
+/* swe_integrals without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same kernels and results as swe_integrals, with the partial sums in a
+ * static device array of the library (24.6 KB per device, any N) instead of
+ * the plan's buffers.  Calls that overlap in time on one device share that
+ * array and race: serialise them (one host thread, or one stream). */
+int swe_integrals_noalloc(swe_plan *plan, swe_state *st, double *mass, double *energy);
+
 /* Serving: continuous batching of independent solves of mixed size and
  * precision under a device-memory budget (no counterpart in the reference,
  * which runs one solve per program).  Requests are queued with a copy of
