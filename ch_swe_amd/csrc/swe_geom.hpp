@@ -30,6 +30,27 @@ __host__ __device__ inline long off(const Dims &d, int p, int i, int j)
     return p * d.PS + (long)(j - 1 + HALO) * d.P + (i - 1 + HALO);
 }
 
+// Device-side addressing of the kernels: 32-bit element offsets (a field
+// holds fewer than 2^32 / 8 elements for N <= 4096) scaled to 32-bit byte
+// offsets from a wave-uniform base pointer, so that a load is one
+// base-plus-offset instruction and arrays indexed alike share the offset
+// register (no 64-bit address arithmetic per access).  Row products use the
+// 24-bit multiplier (indices and pitches stay below 2^24).
+__device__ __forceinline__ unsigned poff(const Dims &d, int p, int i, int j)
+{
+    return (unsigned)p * (unsigned)d.PS + __umul24((unsigned)(j - 1 + HALO), (unsigned)d.P) + (unsigned)(i - 1 + HALO);
+}
+template <typename T>
+__device__ __forceinline__ const T &at(const T *base, unsigned idx)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T &at(T *base, unsigned idx)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (unsigned)sizeof(T));
+}
+
 // flat interior index -> (p, i, j) for a mesh with nx x ny points per panel
 __device__ inline void unflat(long n, int nx, int ny, int &p, int &i, int &j)
 {

@@ -450,6 +450,7 @@ struct Engine : EngineBase {
 
     ~Engine() override
     {
+        if (graph) (void)hipGraphExecDestroy(graph);
         for (void *p : allocs) (void)hipFree(p);
     }
 
@@ -632,15 +633,17 @@ struct Engine : EngineBase {
         timer.launch(k_stage<T, O, TY, 0>, ntiles, nthr, pl->stream, 0, a);
     }
 
+    // nsteps RK4 steps, each: records of v0, then the four stages, each
+    // followed (but the last) by the records of its output
     template <int O>
-    void fused_rk4(T *v0, T *acc, T *ya, T *yb, double dt, int nsteps)
+    void launch_steps(T *v0, T *acc, T *ya, T *yb, double dt, int nsteps)
     {
-        launch_records<O>(v0, rec[0]);
         const T *ins[4] = {v0, ya, yb, ya};
         T *nexts[4] = {ya, yb, ya, nullptr};
         const double A[4] = {0.5 * dt, 0.5 * dt, dt, 0.0};
         const double B[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
-        for (int it = 0; it < nsteps; it++)
+        for (int it = 0; it < nsteps; it++) {
+            launch_records<O>(v0, rec[0]);
             for (int st = 0; st < 4; st++) {
                 StageArgs<T> a = stage_args<O>(ins[st]);
                 a.v0 = v0;
@@ -653,17 +656,60 @@ struct Engine : EngineBase {
                 if (st == 0) timer.launch(k_stage<T, O, TY, 1>, ntiles, nthr, pl->stream, 0, a);
                 else if (st == 3) timer.launch(k_stage<T, O, TY, 3>, ntiles, nthr, pl->stream, 0, a);
                 else timer.launch(k_stage<T, O, TY, 2>, ntiles, nthr, pl->stream, 0, a);
-                // records of the stage output for the next stage (none after
-                // the last stage of the last step)
                 if (st < 3) launch_records<O>(nexts[st], rec[(st + 1) & 1]);
-                else if (it + 1 < nsteps) launch_records<O>(v0, rec[0]);
             }
+        }
+    }
+
+    // The step loop replayed from a captured HIP graph of GRAPH_STEPS steps
+    // (the launches of a step cost more than their gaps otherwise); the graph
+    // is kept for the last (state, registers, dt) it was made for.  Short runs
+    // and timed runs (per-launch events) launch directly.
+    static constexpr int GRAPH_STEPS = 64;
+    hipGraphExec_t graph = nullptr;
+    const void *graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+    double graph_dt = 0;
+
+    template <int O>
+    int fused_rk4(T *v0, T *acc, T *ya, T *yb, double dt, int nsteps)
+    {
+        if (timer.on || nsteps < GRAPH_STEPS) {
+            launch_steps<O>(v0, acc, ya, yb, dt, nsteps);
+            return SWE_OK;
+        }
+        const void *key[4] = {v0, acc, ya, yb};
+        if (!graph || std::memcmp(key, graph_key, sizeof key) != 0 || graph_dt != dt) {
+            if (graph) (void)hipGraphExecDestroy(graph);
+            graph = nullptr;
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal));
+            launch_steps<O>(v0, acc, ya, yb, dt, GRAPH_STEPS);
+            const hipError_t le = hipGetLastError();
+            const hipError_t ce = hipStreamEndCapture(pl->stream, &g);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(SWE_EHIP, std::string("step graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+            }
+            const hipError_t ie = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                graph = nullptr;
+                return fail(SWE_EHIP, std::string("step graph instantiate: ") + hipGetErrorString(ie));
+            }
+            std::memcpy(graph_key, key, sizeof key);
+            graph_dt = dt;
+        }
+        int done = 0;
+        for (; done + GRAPH_STEPS <= nsteps; done += GRAPH_STEPS) HIP_TRY(hipGraphLaunch(graph, pl->stream));
+        launch_steps<O>(v0, acc, ya, yb, dt, nsteps - done);
+        return SWE_OK;
     }
 
     int rk4(void *v0, void *acc, void *ya, void *yb, double dt, int nsteps) override
     {
-        if (order == 21) fused_rk4<21>((T *)v0, (T *)acc, (T *)ya, (T *)yb, dt, nsteps);
-        else fused_rk4<42>((T *)v0, (T *)acc, (T *)ya, (T *)yb, dt, nsteps);
+        const int rc = order == 21 ? fused_rk4<21>((T *)v0, (T *)acc, (T *)ya, (T *)yb, dt, nsteps)
+                                   : fused_rk4<42>((T *)v0, (T *)acc, (T *)ya, (T *)yb, dt, nsteps);
+        if (rc) return rc;
         HIP_TRY(hipGetLastError());
         return SWE_OK;
     }

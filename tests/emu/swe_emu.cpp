@@ -262,6 +262,13 @@ void launch_flat(unsigned grid, unsigned nt, const std::function<void()> &fn)
     g_flat = false;
 }
 
+// the 24-bit multiplier of the device: both operands must fit in 24 bits
+unsigned umul24(unsigned a, unsigned b)
+{
+    if (a >= (1u << 24) || b >= (1u << 24)) die("__umul24 operand beyond 24 bits");
+    return a * b;
+}
+
 void release()
 {
     for (auto &f : fib) std::free(f.stack);
@@ -279,6 +286,7 @@ void release()
 #define __shfl(v, s) emu::shfl((v), (s))
 #define __shfl_down(v, d) emu::shfl_down((v), (d))
 #define __builtin_amdgcn_s_setprio(x) ((void)0)
+#define __umul24(a, b) emu::umul24((a), (b))
 #undef __shared__
 #define __shared__ static
 #undef __launch_bounds__

@@ -275,3 +275,31 @@ def test_duplicated_h_stays_bitwise_continuous(oracle_mod, precision, order, N):
     st = op.create_stvec().upload(u * 1e-3, v * 1e-3, h)
     cs.create_timescheme("rk4").step(st, op, 600.0, 20)
     assert _dup_mismatch(N, order, st.download()[2]) == 0.0
+
+
+@pytest.mark.parametrize("precision", [cs.FP64, cs.FP32])
+def test_rk4_graph_replays_match_direct_launches(oracle_mod, precision):
+    """swe_rk4_steps replays its step loop from a captured graph for runs of
+    64 steps or more: three 64-step calls (graph replays, with downloads and
+    other work between them) give bitwise what 8-step calls (direct
+    launches) give, after every call; FP64 also matches the oracle."""
+    N, order, dt = 40, 42, 900.0
+    _say(f"rk4 graph replay precision={precision}")
+    o = oracle_mod.Oracle(N, order)
+    s0 = _random_state(o, 77)
+    op = _op(N, order, precision)
+    rk4 = cs.create_timescheme("rk4")
+    a = op.create_stvec().upload(*s0)
+    b = op.create_stvec().upload(*s0)
+    for rep in range(3):
+        rk4.step(a, op, dt, 64)
+        for _ in range(8):
+            rk4.step(b, op, dt, 8)
+        other = op.create_stvec().upload(*s0)  # eager work between replays
+        op.apply(op.create_stvec(), other)
+        for g, r in zip(a.download(), b.download()):
+            assert np.array_equal(g, r), rep
+    if precision == cs.FP64:
+        ref = o.rk4(*s0, dt, 192)
+        for g, r in zip(a.download(), ref):
+            assert _rel(g, r) < 1e-12
