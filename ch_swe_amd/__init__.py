@@ -134,7 +134,7 @@ def _p(a):
 
 
 # swe_config.flags and swe_ensemble_rk4 flags (include/ch_swe.h)
-FLAG_UNFUSED, FLAG_LINEAR_BLOCKS, FLAG_GENERAL_TILES = 1, 2, 4
+FLAG_UNFUSED, FLAG_LINEAR_BLOCKS, FLAG_GENERAL_TILES, FLAG_COMPENSATED = 1, 2, 4, 8
 ENS_GRAPH = 1
 
 
@@ -146,8 +146,9 @@ class Config:
     def __init__(self, N, order=42, precision=FP64, radius=6.371e6, grav=9.81,
                  h_mean=1000.0, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
                  coriolis_const=False, unfused=False, linear_blocks=False, block_threads=0,
-                 general_tiles=False):
+                 general_tiles=False, compensated=False):
         self.N, self.order, self.precision = int(N), int(order), int(precision)
+        self.compensated = bool(compensated)
         self.general_tiles = bool(general_tiles)
         self.block_threads = int(block_threads)
         self.unfused = bool(unfused)
@@ -163,7 +164,7 @@ class Config:
         c.radius, c.grav, c.h_mean, c.omega = self.radius, self.grav, self.h_mean, self.omega
         c.axis[:] = self.axis
         c.flags = ((FLAG_UNFUSED if self.unfused else 0) | (FLAG_LINEAR_BLOCKS if self.linear_blocks else 0) |
-                   (FLAG_GENERAL_TILES if self.general_tiles else 0))
+                   (FLAG_GENERAL_TILES if self.general_tiles else 0) | (FLAG_COMPENSATED if self.compensated else 0))
         c.block_threads = self.block_threads
         return c
 
@@ -278,15 +279,17 @@ class Rk4:
 def create_linear_swm_operator(N, order=42, precision=FP64, grav=9.81, h_mean=1000.0,
                                radius=6.371e6, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
                                coriolis_const=False, unfused=False, linear_blocks=False, block_threads=0,
-                               general_tiles=False):
+                               general_tiles=False, compensated=False):
     """Factory mirroring create_linear_swm_operator
     (operator_swm_factory_mod.f90 L586-685) for the Ch staggering.
     unfused=True selects the multi-kernel diagnostic path; linear_blocks=True
     the plain block order of the fused stage kernel; general_tiles=True sends
     every tile through its general path; block_threads (0 or 512) its
-    workgroup size."""
+    workgroup size; compensated=True runs RK4 with compensated summation of
+    each step's state update (states carry a compensation field)."""
     return OperatorSwmLin(Config(N, order, precision, radius, grav, h_mean, omega, axis,
-                                 coriolis_const, unfused, linear_blocks, block_threads, general_tiles))
+                                 coriolis_const, unfused, linear_blocks, block_threads, general_tiles,
+                                 compensated))
 
 
 class Ensemble:

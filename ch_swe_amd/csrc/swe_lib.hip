@@ -635,9 +635,13 @@ struct Engine : EngineBase {
 
     // nsteps RK4 steps, each: records of v0, then the four stages, each
     // followed (but the last) by the records of its output
+    // compensation field of state v0 (compensated plans), else nullptr
+    T *cmp_of(T *v0) const { return (pl->cfg.flags & SWE_FLAG_COMPENSATED) ? v0 + 3 * d.F : nullptr; }
+
     template <int O>
     void launch_steps(T *v0, T *acc, T *ya, T *yb, double dt, int nsteps)
     {
+        T *cmp = cmp_of(v0);
         const T *ins[4] = {v0, ya, yb, ya};
         T *nexts[4] = {ya, yb, ya, nullptr};
         const double A[4] = {0.5 * dt, 0.5 * dt, dt, 0.0};
@@ -650,10 +654,12 @@ struct Engine : EngineBase {
                 a.acc = acc;
                 a.ynext = nexts[st];
                 a.out = v0;
+                a.cmp = cmp;
                 a.a_dt = T(A[st]);
                 a.b_dt = T(B[st]);
                 a.rec_in = rec[st & 1];
                 if (st == 0) timer.launch(k_stage<T, O, TY, 1>, ntiles, nthr, pl->stream, 0, a);
+                else if (st == 3 && cmp) timer.launch(k_stage<T, O, TY, 4>, ntiles, nthr, pl->stream, 0, a);
                 else if (st == 3) timer.launch(k_stage<T, O, TY, 3>, ntiles, nthr, pl->stream, 0, a);
                 else timer.launch(k_stage<T, O, TY, 2>, ntiles, nthr, pl->stream, 0, a);
                 if (st < 3) launch_records<O>(nexts[st], rec[(st + 1) & 1]);
@@ -832,8 +838,10 @@ int validate_config(const swe_config *cfg, const char *who)
         return fail(SWE_EINVAL, w + "precision must be SWE_FP64 or SWE_FP32");
     if (cfg->block_threads != 0 && cfg->block_threads != F_THREADS)
         return fail(SWE_EINVAL, w + "block_threads must be 0 or 512 (the fused kernels' workgroup size)");
-    if (cfg->flags & ~(SWE_FLAG_UNFUSED | SWE_FLAG_LINEAR_BLOCKS | SWE_FLAG_GENERAL_TILES))
+    if (cfg->flags & ~(SWE_FLAG_UNFUSED | SWE_FLAG_LINEAR_BLOCKS | SWE_FLAG_GENERAL_TILES | SWE_FLAG_COMPENSATED))
         return fail(SWE_EINVAL, w + "unknown bits in flags");
+    if ((cfg->flags & SWE_FLAG_COMPENSATED) && (cfg->flags & SWE_FLAG_UNFUSED))
+        return fail(SWE_EINVAL, w + "compensated summation runs on the fused path (not with SWE_FLAG_UNFUSED)");
     if (!(cfg->radius > 0) || !std::isfinite(cfg->radius) || !std::isfinite(cfg->grav) ||
         !std::isfinite(cfg->h_mean) || !std::isfinite(cfg->omega) || !std::isfinite(cfg->axis[0]) ||
         !std::isfinite(cfg->axis[1]) || !std::isfinite(cfg->axis[2]))
@@ -871,6 +879,9 @@ int upload_t(swe_state *st, const double *u, const double *v, const double *h)
     pack_to_padded(d, h, N + 1, N + 1, host.data() + 2 * d.F);
     HIP_TRY(hipMemcpyAsync(st->buf, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice,
                            st->plan->stream));
+    // a new value carries no compensation
+    if (st->plan->cfg.flags & SWE_FLAG_COMPENSATED)
+        HIP_TRY(hipMemsetAsync((T *)st->buf + 3 * d.F, 0, 3 * d.F * sizeof(T), st->plan->stream));
     HIP_TRY(hipStreamSynchronize(st->plan->stream));
     return SWE_OK;
 }
@@ -970,7 +981,9 @@ int swe_synchronize(swe_plan *plan)
 int swe_state_create(swe_plan *plan, swe_state **out)
 {
     if (!plan || !out) return fail(SWE_EINVAL, "swe_state_create: NULL argument");
-    size_t bytes = 3 * plan->d.F * plan->eng->elem();
+    // u, v, h, and with compensated summation their compensation fields
+    const size_t nfields = (plan->cfg.flags & SWE_FLAG_COMPENSATED) ? 6 : 3;
+    size_t bytes = nfields * plan->d.F * plan->eng->elem();
     void *buf = nullptr;
     hipError_t e = hipMalloc(&buf, bytes);
     if (e != hipSuccess) return fail(SWE_ENOMEM, std::string("hipMalloc(state): ") + hipGetErrorString(e));
@@ -1016,6 +1029,12 @@ static int axpy_common(swe_plan *plan, swe_state *out, double s1, swe_state *v1,
     int rc;
     if ((rc = check_state(plan, out, "out")) || (rc = check_state(plan, v1, "v1"))) return rc;
     if (v2 && (rc = check_state(plan, v2, "v2"))) return rc;
+    // assign gives out a new value, which carries no compensation; update
+    // adds to the value v0 + cmp stands for, so its compensation stays
+    if (!update && (plan->cfg.flags & SWE_FLAG_COMPENSATED)) {
+        const size_t n = 3 * plan->d.F * plan->eng->elem();
+        HIP_TRY(hipMemsetAsync((char *)out->buf + n, 0, n, plan->stream));
+    }
     return plan->eng->axpy(out->buf, s1, v1->buf, s2, v2 ? v2->buf : nullptr, update);
 }
 

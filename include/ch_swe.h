@@ -39,7 +39,9 @@ enum { SWE_FP64 = 0, SWE_FP32 = 1 };
 enum {
     SWE_FLAG_UNFUSED = 1,      /* run the unfused multi-kernel path (diagnostics) */
     SWE_FLAG_LINEAR_BLOCKS = 2, /* plain block order instead of the XCD-aware one */
-    SWE_FLAG_GENERAL_TILES = 4  /* diagnostic: every tile of the fused stage takes the general path */
+    SWE_FLAG_GENERAL_TILES = 4, /* diagnostic: every tile of the fused stage takes the general path */
+    SWE_FLAG_COMPENSATED = 8    /* RK4 with compensated (Kahan) summation of each step's update of the
+                                   state: states carry a compensation field (twice the device memory) */
 };
 
 /* swe_ensemble_rk4 flags */

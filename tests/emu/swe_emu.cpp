@@ -23,7 +23,8 @@
 // usage: swe_emu <case> N order prec [steps] [seed] [flags]
 //   case  : apply | rk4 | integrals | multi
 //   prec  : 64 | 32
-//   flags : bit 0 every tile on the general path, bit 1 plain block order
+//   flags : bit 0 every tile on the general path, bit 1 plain block order,
+//           bit 2 compensated summation (rk4)
 // Prints one line per field and exits 0 when every field is within the bar.
 #include <hip/hip_runtime.h>
 #include <setjmp.h>
@@ -357,10 +358,14 @@ struct Plan {
         }
     }
 
-    // a state: padded layout, NaN outside the interior points
+    // a state: padded layout, NaN outside the interior points; with
+    // compensated summation (flag bit 2) followed by its compensation
+    // fields, zero
     T *state()
     {
-        return dev(pool, 3 * (size_t)d.F, T(NAN));
+        T *s = dev(pool, (flags & 4 ? 6 : 3) * (size_t)d.F, T(NAN));
+        if (flags & 4) std::fill(s + 3 * d.F, s + 6 * d.F, T(0));
+        return s;
     }
     void upload(T *st, const double *u, const double *v, const double *h)
     {
@@ -432,6 +437,7 @@ struct Plan {
     void rk4(T *v0, double dt, int nsteps)
     {
         T *acc = state(), *ya = state(), *yb = state();
+        T *cmp = (flags & 4) ? v0 + 3 * d.F : nullptr;
         records<O>(v0, rec[0]);
         const T *ins[4] = {v0, ya, yb, ya};
         T *nexts[4] = {ya, yb, ya, nullptr};
@@ -444,10 +450,12 @@ struct Plan {
                 a.acc = acc;
                 a.ynext = nexts[st];
                 a.out = v0;
+                a.cmp = cmp;
                 a.a_dt = T(A[st]);
                 a.b_dt = T(B[st]);
                 a.rec_in = rec[st & 1];
                 if (st == 0) stage<O, 1>(a);
+                else if (st == 3 && cmp) stage<O, 4>(a);
                 else if (st == 3) stage<O, 3>(a);
                 else stage<O, 2>(a);
                 if (st < 3) records<O>(nexts[st], rec[(st + 1) & 1]);

@@ -6,7 +6,7 @@ blocks in a random order, every device buffer an exact-size heap allocation
 (AddressSanitizer reports any access outside it) and the state padding NaN.
 The cases cover single-tile panels, uneven last tiles, interior tiles, the
 general-tiles and plain-order flags, both SBP orders and precisions, the RK4
-stage modes, the multi-instance (ensemble) kernels and the tile-wise
+stage modes (compensated summation included), the multi-instance (ensemble) kernels and the tile-wise
 integrals.  No GPU is used."""
 import os
 import shutil
@@ -28,6 +28,8 @@ CASES = [
     ("apply", 40, 42, 32, 1, 6, 2),     # plain block order
     ("rk4", 10, 42, 64, 2, 7, 0),       # stage modes 1-3, two steps
     ("rk4", 50, 21, 32, 1, 8, 0),
+    ("rk4", 12, 42, 64, 3, 13, 4),      # compensated last stage (mode 4)
+    ("rk4", 40, 21, 32, 2, 14, 4),
     ("integrals", 20, 42, 64, 1, 9, 0),
     ("integrals", 48, 21, 32, 1, 10, 0),
     ("multi", 12, 42, 64, 2, 11, 0),    # two members (N, N + 7), own dt and steps

@@ -105,3 +105,27 @@ def test_ensemble_rejects_bad_members():
         ens.rk4(100.0, -1)
     with pytest.raises(cs.SweError, match="finite"):
         ens.rk4(float("nan"), 1)
+
+
+def test_compensated_members_match_alone():
+    """Members with compensated summation (plans with SWE_FLAG_COMPENSATED)
+    advance exactly as alone; an ensemble refuses to mix them with plain
+    members."""
+    print("ensemble compensated", flush=True)
+    rng = np.random.default_rng(9)
+    N, order, dt, steps = 24, 42, 1500.0, 70
+    op = cs.create_linear_swm_operator(N, order, cs.FP32, compensated=True)
+    s0 = [(rng.standard_normal((6, N + 1, N)), rng.standard_normal((6, N, N + 1)),
+           rng.standard_normal((6, N + 1, N + 1))) for _ in range(3)]
+    mem = [op.create_stvec().upload(*s) for s in s0]
+    ens = cs.Ensemble(mem)
+    ens.rk4(dt, steps, graph=True)
+    ens.synchronize()
+    for k in range(3):
+        alone = op.create_stvec().upload(*s0[k])
+        cs.create_timescheme("rk4").step(alone, op, dt, steps)
+        for g, a in zip(mem[k].download(), alone.download()):
+            assert np.array_equal(g, a), k
+    plain = cs.create_linear_swm_operator(N, order, cs.FP32)
+    with pytest.raises(cs.SweError, match="compensated"):
+        cs.Ensemble([op.create_stvec(), plain.create_stvec()])

@@ -303,3 +303,43 @@ def test_rk4_graph_replays_match_direct_launches(oracle_mod, precision):
         ref = o.rk4(*s0, dt, 192)
         for g, r in zip(a.download(), ref):
             assert _rel(g, r) < 1e-12
+
+
+def test_compensated_rk4_fp64_matches_oracle(oracle_mod):
+    """Compensated summation of the step update keeps FP64 at the oracle's
+    trajectory (1e-12), through the graph-replayed loop too."""
+    N, order, dt = 24, 42, 1200.0
+    _say("compensated rk4 fp64")
+    o = oracle_mod.Oracle(N, order)
+    s0 = _random_state(o, 31)
+    op = _op(N, order, cs.FP64, compensated=True)
+    st = op.create_stvec().upload(*s0)
+    cs.create_timescheme("rk4").step(st, op, dt, 70)
+    for g, r in zip(st.download(), o.rk4(*s0, dt, 70)):
+        assert _rel(g, r) < 1e-12
+
+
+def test_compensated_state_semantics():
+    """assign gives a compensated state a fresh value (no stale
+    compensation): stepping it equals stepping a freshly uploaded copy,
+    bitwise; upload resets the compensation too."""
+    N, order, dt = 20, 21, 900.0
+    _say("compensated assign/upload semantics")
+    rng = np.random.default_rng(3)
+    s0 = (rng.standard_normal((6, N + 1, N)), rng.standard_normal((6, N, N + 1)),
+          rng.standard_normal((6, N + 1, N + 1)))
+    op = _op(N, order, cs.FP32, compensated=True)
+    rk4 = cs.create_timescheme("rk4")
+    a = op.create_stvec().upload(*s0)
+    rk4.step(a, op, dt, 30)             # a now carries a compensation
+    b = op.create_stvec().upload(*s0)
+    a.assign(1.0, b)                     # fresh value again
+    rk4.step(a, op, dt, 30)
+    c = op.create_stvec().upload(*s0)
+    rk4.step(c, op, dt, 30)
+    for g, r in zip(a.download(), c.download()):
+        assert np.array_equal(g, r)
+    a.upload(*s0)
+    rk4.step(a, op, dt, 30)
+    for g, r in zip(a.download(), c.download()):
+        assert np.array_equal(g, r)
