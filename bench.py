@@ -145,7 +145,7 @@ def bench_ensemble(args, ev):
     import ch_swe_amd as cs
 
     N, members, steps = 96, args.members, args.steps
-    states = tc.ensemble_states(N, ORDER, members, lmax=16, seed=1)
+    states = tc.ensemble_states(N, ORDER, members, lmax=getattr(args, "lmax", 16), seed=1)
     dt = tc.cfl_dt(N, 0.5)
     op = cs.create_linear_swm_operator(N, ORDER, cs.FP32)
     mem = [op.create_stvec().upload(*s) for s in states]
@@ -166,6 +166,16 @@ def bench_ensemble(args, ev):
     solo = op.create_stvec().upload(*states[0])
     cs.create_timescheme("rk4").step(solo, op, dt, args.warmup + steps)
     d = max(float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-30)) for x, y in zip(mem[0].download(), solo.download()))
+    # per-member FP32-vs-FP64 error: an FP64 ensemble of the first members, same steps
+    n64 = min(4, members)
+    op64 = cs.create_linear_swm_operator(N, ORDER, cs.FP64)
+    mem64 = [op64.create_stvec().upload(*states[k]) for k in range(n64)]
+    ens64 = cs.Ensemble(mem64)
+    ens64.rk4(dt, args.warmup + steps, graph=True)
+    ens64.synchronize()
+    w = {m: tc.sbp_weights(N, m, ORDER) for m in ("u", "v", "h")}
+    member_err = [{m: float(f"{rel_l2(a, b, w[m]):.3e}") for m, a, b in zip("uvh", mem[k].download(), mem64[k].download())}
+                  for k in range(n64)]
     return {
         "metric": "aggregate Mpts*steps/s, uniform batched ensemble (FP32 stencils)",
         "value": round(value, 2), "unit": "Mpts*steps/s", "n_gpus": 1, "steps": steps,
@@ -175,6 +185,8 @@ def bench_ensemble(args, ev):
         "config": {"workload": f"uniform ensemble: {members} members, N={N}, sbp42, RK4 at CFL 0.5, HIP-graph step loop",
                    "members": members, "N": N, "order": ORDER, "dt_s": round(dt, 4)},
         "member0_vs_alone_max_rel_diff": d,
+        "member_rel_l2_fp32_vs_fp64": member_err,
+        "after_steps": args.warmup + steps,
     }
 
 
@@ -198,7 +210,7 @@ def bench_uneven(args, ev):
         key = (int(Ns[i]), int(orders[i]))
         if key not in plans:
             plans[key] = cs.create_linear_swm_operator(key[0], key[1], cs.FP32)
-        init.append(tc.ensemble_states(key[0], key[1], 1, lmax=16, seed=100 + i, basis=1)[0])
+        init.append(tc.ensemble_states(key[0], key[1], 1, lmax=getattr(args, "lmax", 16), seed=100 + i, basis=1)[0])
     cells = np.array([6 * int(N) * int(N) for N in Ns])
     work = float(np.sum(cells * steps))
     # batched
@@ -227,6 +239,24 @@ def bench_uneven(args, ev):
     ms_seq = (time.perf_counter() - t0) * 1e3
     d = max(float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-30))
             for i in range(n_inst) for x, y in zip(mem[i].download(), solo[i].download()))
+    # per-instance FP32-vs-FP64 error: FP64 copies of a sample of the instances, batched
+    sample = list(range(min(8, n_inst)))
+    plans64 = {}
+    mem64 = []
+    for i in sample:
+        key = (int(Ns[i]), int(orders[i]))
+        if key not in plans64:
+            plans64[key] = cs.create_linear_swm_operator(key[0], key[1], cs.FP64)
+        mem64.append(plans64[key].create_stvec().upload(*init[i]))
+    ens64 = cs.Ensemble(mem64)
+    ens64.rk4(dts[sample], steps[sample], graph=True)
+    ens64.synchronize()
+    inst_err = []
+    for j, i in enumerate(sample):
+        N_i, o_i = int(Ns[i]), int(orders[i])
+        w = {m: tc.sbp_weights(N_i, m, o_i) for m in ("u", "v", "h")}
+        e = {m: float(f"{rel_l2(a, b, w[m]):.3e}") for m, a, b in zip("uvh", mem[i].download(), mem64[j].download())}
+        inst_err.append({"N": N_i, "order": o_i, "steps": int(steps[i]), **e})
     return {
         "metric": "aggregate Mpts*steps/s, uneven batch (FP32 stencils)",
         "value": round(work / (ms_batch * 1e-3) / 1e6, 2), "unit": "Mpts*steps/s", "n_gpus": 1,
@@ -240,6 +270,7 @@ def bench_uneven(args, ev):
         "one_at_a_time_value": round(work / (ms_seq * 1e-3) / 1e6, 2),
         "speedup_vs_one_at_a_time": round(ms_seq / ms_batch, 2),
         "batched_vs_alone_max_rel_diff": d,
+        "instance_rel_l2_fp32_vs_fp64": inst_err,
     }
 
 
@@ -254,7 +285,7 @@ def bench_serving(args, ev):
     import ch_swe_amd as cs
 
     sizes, weights = [32, 64, 128, 256, 512], [0.3, 0.3, 0.2, 0.15, 0.05]
-    base = {N: tc.sh_state(N, ORDER, lmax=16, seed=7, rms=20.0, workers=1) for N in sizes}
+    base = {N: tc.sh_state(N, ORDER, lmax=getattr(args, "lmax", 16), seed=7, rms=20.0, workers=1) for N in sizes}
     geo = {N: tc.mesh_geometry(N, "h")[0] for N in sizes}
     rates = [float(r) for r in args.rates.split(",")]
     n_req = args.members or 48
@@ -318,6 +349,80 @@ def bench_serving(args, ev):
     }
 
 
+def accuracy_leg(cs, tc, N, state0, dt, steps):
+    """FP32 (plain and with compensated summation) against FP64 after
+    `steps` RK4 steps from the benchmark state, untimed: relative SBP-L2
+    error of u, v, h and the relative drift of the linear energy."""
+    w = {m: tc.sbp_weights(N, m, ORDER) for m in ("u", "v", "h")}
+    res = {}
+    for name, prec, comp in (("fp64", cs.FP64, False), ("fp32", cs.FP32, False), ("fp32_compensated", cs.FP32, True)):
+        op = cs.create_linear_swm_operator(N, ORDER, prec, compensated=comp)
+        st = op.create_stvec().upload(*state0)
+        _, e0 = op.integrals(st)
+        cs.create_timescheme("rk4").step(st, op, dt, steps)
+        _, e1 = op.integrals(st)
+        res[name] = (st.download(), e1 / e0 - 1.0)
+    out = {"after_steps": steps, "energy_drift_fp64": float(f"{res['fp64'][1]:.3e}")}
+    for name in ("fp32", "fp32_compensated"):
+        out["rel_l2_" + name + "_vs_fp64"] = {m: float(f"{rel_l2(a, b, w[m]):.3e}")
+                                             for m, a, b in zip("uvh", res[name][0], res["fp64"][0])}
+        out["energy_drift_" + name] = float(f"{res[name][1]:.3e}")
+    return out
+
+
+def oracle_check(cs, N, state0, dt, steps, threads):
+    """The GPU path against the CPU oracle on the benchmark state, `steps`
+    RK4 steps: FP64 max relative error (bar 1e-12) and FP32 relative L2."""
+    from ch_swe_amd import testcases as tc
+    from oracle import swe_oracle
+
+    swe_oracle.set_num_threads(threads)
+    ref = swe_oracle.Oracle(N, ORDER).rk4(*state0, dt, steps)
+    w = {m: tc.sbp_weights(N, m, ORDER) for m in ("u", "v", "h")}
+    out = {"steps": steps}
+    for name, prec in (("fp64", cs.FP64), ("fp32", cs.FP32)):
+        op = cs.create_linear_swm_operator(N, ORDER, prec)
+        st = op.create_stvec().upload(*state0)
+        cs.create_timescheme("rk4").step(st, op, dt, steps)
+        got = st.download()
+        if prec == cs.FP64:
+            out["fp64_max_rel_err"] = float(f"{max(np.abs(g - r).max() / np.abs(r).max() for g, r in zip(got, ref)):.3e}")
+        else:
+            out["fp32_rel_l2"] = {m: float(f"{rel_l2(g, r, w[m]):.3e}") for m, g, r in zip("uvh", got, ref)}
+    return out
+
+
+def sample_workloads(args, ev):
+    """Small versions of the batch and serving workloads, run after the
+    headline measurement: the uniform ensemble (256 x N=96, 100 timed
+    steps), the uneven batch (64 instances, step counts scaled by 1/4) and
+    the serving loop at 10, 100 and 500 requests/s (24 requests each).  The
+    host-side initial states use spherical harmonics of degree 6 instead of
+    16 (generation time; the solves' cost does not depend on it)."""
+    import copy
+
+    out = {}
+    a = copy.copy(args)
+    a.members, a.steps, a.warmup, a.lmax = 256, 100, 10, 6
+    e = bench_ensemble(a, ev)
+    out["ensemble"] = {k: e[k] for k in ("value", "unit", "ms_per_step", "member_rel_l2_fp32_vs_fp64", "after_steps",
+                                         "member0_vs_alone_max_rel_diff")}
+    out["ensemble"]["config"] = e["config"]["workload"]
+    print(f"# ensemble sample: {e['value']} Mpts*steps/s", file=sys.stderr, flush=True)
+    a = copy.copy(args)
+    a.members, a.steps_scale, a.lmax = 64, 0.25, 6
+    u = bench_uneven(a, ev)
+    out["uneven"] = {k: u[k] for k in ("value", "unit", "one_at_a_time_value", "speedup_vs_one_at_a_time",
+                                       "batched_vs_alone_max_rel_diff", "instance_rel_l2_fp32_vs_fp64")}
+    out["uneven"]["config"] = u["config"]["workload"] + " (step counts x 0.25)"
+    print(f"# uneven sample: {u['value']} Mpts*steps/s", file=sys.stderr, flush=True)
+    a = copy.copy(args)
+    a.members, a.rates, a.lmax = 24, "10,100,500", 6
+    v = bench_serving(a, ev)
+    out["serving"] = {"per_rate": v["per_rate"], "config": v["config"]["workload"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,6 +439,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=25, help="serving: steps per tick")
     ap.add_argument("--members", type=int, default=None, help="ensemble members / batch instances")
     ap.add_argument("--steps-scale", type=float, default=1.0, help="uneven batch: scale every step count")
+    ap.add_argument("--no-samples", action="store_true",
+                    help="single workload: skip the accuracy, oracle and batch/serving sample legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (3 launches) so that a "
                          "counter profile can scale FETCH_SIZE / WRITE_SIZE by a known size")
@@ -419,6 +526,14 @@ def main():
     one_steps = max(1, args.cpu_steps // 10)
     cpu1_val, cpu1_sec, _ = (cpu_baseline(N, state0, dt, one_steps, 1) if args.cpu_steps > 0 else (None, 0.0, 0))
 
+    extra = {}
+    if not args.no_samples:
+        extra["accuracy_1000_steps"] = accuracy_leg(cs, tc, N, state0, dt, 1000)
+        print(f"# accuracy leg: {extra['accuracy_1000_steps']}", file=sys.stderr, flush=True)
+        extra["oracle_check"] = oracle_check(cs, N, state0, dt, 20, args.cpu_threads)
+        print(f"# oracle check: {extra['oracle_check']}", file=sys.stderr, flush=True)
+        extra.update(sample_workloads(args, ev))
+
     line = {
         "metric": "Mpts*steps/s (FP32 stencils, FP64 metric and integrals), N=384 sbp42 single solve",
         "value": round(value, 2),
@@ -457,8 +572,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None if traffic is None else traffic["hbm_bytes_per_launch_mean"],
             "traffic_source": None if traffic is None else
-            "rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 512 MiB copy), mean of the stage "
-            "launches of a step, profiles/" + TRAFFIC_FILE,
+            "rocprofv3 TCC_EA0 request counts by size (32/64/128 B reads, 32/64 B writes; a 512 MiB copy "
+            "counts 512 MiB each way), mean of the stage launches of a step, profiles/" + TRAFFIC_FILE,
         },
         "cpu_baseline": {
             "value": None if cpu_val is None else round(cpu_val, 3),
@@ -471,6 +586,7 @@ def main():
             "single_core_sample": f"the same oracle on 1 thread, {one_steps} RK4 steps ({cpu1_sec:.1f} s)",
         },
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 
