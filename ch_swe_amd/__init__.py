@@ -26,7 +26,7 @@ import numpy as np
 __all__ = [
     "SweError", "lib", "Config", "OperatorSwmLin", "StvecSwm", "Rk4",
     "create_linear_swm_operator", "create_timescheme", "halo_map", "Ensemble", "Server",
-    "FP64", "FP32", "LIB_PATH",
+    "FP64", "FP32", "LIB_PATH", "GenericConfig", "parse_config_file",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -45,6 +45,9 @@ EXPORTS = (
     "swe_server_counters",
     "swe_check_tables",
 )
+# the configuration reader's entry points (include/ch_swe_config.h)
+CFG_EXPORTS = ("swe_cfg_parse", "swe_cfg_read", "swe_cfg_destroy", "swe_cfg_has", "swe_cfg_get_real",
+               "swe_cfg_get_int", "swe_cfg_get_logical", "swe_cfg_get_str")
 
 
 class SweError(RuntimeError):
@@ -119,6 +122,14 @@ def lib():
     L.swe_server_take.argtypes = [_vp, ctypes.c_longlong, _dp, _dp, _dp, _ip]
     L.swe_server_stats.argtypes = [_vp, _ip, _ip, ctypes.POINTER(ctypes.c_size_t)]
     L.swe_server_counters.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 3
+    L.swe_cfg_parse.argtypes = [ctypes.c_char_p, ctypes.POINTER(_vp)]
+    L.swe_cfg_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(_vp)]
+    L.swe_cfg_destroy.argtypes = [_vp]
+    L.swe_cfg_has.argtypes = [_vp, ctypes.c_char_p]
+    L.swe_cfg_get_real.argtypes = [_vp, ctypes.c_char_p, _dp]
+    L.swe_cfg_get_int.argtypes = [_vp, ctypes.c_char_p, _ip]
+    L.swe_cfg_get_logical.argtypes = [_vp, ctypes.c_char_p, _ip]
+    L.swe_cfg_get_str.argtypes = [_vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _ip]
     _lib = L
     return L
 
@@ -403,6 +414,76 @@ class Server:
         x = [ctypes.c_longlong() for _ in range(3)]
         _check(lib().swe_server_counters(self.h, *[ctypes.byref(v) for v in x]))
         return tuple(v.value for v in x)
+
+
+class GenericConfig:
+    """A parsed configuration in the reference's format, read through
+    libchswe.so (include/ch_swe_config.h): parse_config_file
+    (src/stuff/config/config_tools_mod.f90 L12-26) and generic_config_t%get
+    (abstract_generic_config_mod.f90).  Keys are paths ("domain%metric%omega");
+    arrays read back with get_array.  A missing key raises SweError unless a
+    default is given; so does an entry of another type."""
+
+    _MISSING = object()
+
+    def __init__(self, text=None, path=None):
+        L = lib()
+        h = _vp()
+        if (text is None) == (path is None):
+            raise ValueError("give exactly one of text and path")
+        if text is not None:
+            _check(L.swe_cfg_parse(text.encode(), ctypes.byref(h)))
+        else:
+            _check(L.swe_cfg_read(os.fspath(path).encode(), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().swe_cfg_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def has(self, key):
+        return lib().swe_cfg_has(self._h, key.encode()) == 1
+
+    def _get(self, key, default, fn, cval, conv):
+        if default is not self._MISSING and not self.has(key):
+            return default
+        _check(fn(self._h, key.encode(), ctypes.byref(cval)))
+        return conv(cval.value)
+
+    def get_real(self, key, default=_MISSING):
+        return self._get(key, default, lib().swe_cfg_get_real, ctypes.c_double(), float)
+
+    def get_int(self, key, default=_MISSING):
+        return self._get(key, default, lib().swe_cfg_get_int, ctypes.c_int(), int)
+
+    def get_logical(self, key, default=_MISSING):
+        return self._get(key, default, lib().swe_cfg_get_logical, ctypes.c_int(), bool)
+
+    def get_str(self, key, default=_MISSING):
+        if default is not self._MISSING and not self.has(key):
+            return default
+        n = ctypes.c_int()
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().swe_cfg_get_str(self._h, key.encode(), buf, len(buf), ctypes.byref(n)))
+        if n.value >= len(buf):
+            buf = ctypes.create_string_buffer(n.value + 1)
+            _check(lib().swe_cfg_get_str(self._h, key.encode(), buf, len(buf), ctypes.byref(n)))
+        return buf.value.decode()
+
+    def get_array(self, key, default=_MISSING):
+        """A one-level array entry as a list of reals (name%1 ... name%n)."""
+        if default is not self._MISSING and not self.has(key + "%length"):
+            return default
+        n = self.get_int(key + "%length")
+        return [self.get_real(f"{key}%{i}") for i in range(1, n + 1)]
+
+
+def parse_config_file(path):
+    """parse_config_file (config_tools_mod.f90 L12-26)."""
+    return GenericConfig(path=path)
 
 
 def create_timescheme(name):

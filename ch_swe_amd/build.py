@@ -9,7 +9,7 @@ LIB = os.path.join(HERE, "libchswe.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["swe_lib.hip", "swe_topology.cpp"]
+SOURCES = ["swe_lib.hip", "swe_topology.cpp", "swe_config.cpp"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep the compiler from forming packed FP32 instructions
@@ -20,8 +20,8 @@ FLAGS = [
 
 FORTRAN = os.path.join(HERE, "fortran")
 FC = os.environ.get("FC_FLANG", "/opt/rocm/lib/llvm/bin/flang")
-FORTRAN_SOURCES = ["swe_hip_capi_mod.f90", "swe_hip_mod.f90"]
-FORTRAN_PROGRAMS = {"swe_hip_gaussian": "swe_hip_gaussian_main.f90"}
+FORTRAN_SOURCES = ["swe_hip_capi_mod.f90", "swe_hip_mod.f90", "swe_hip_cfg_mod.f90"]
+FORTRAN_PROGRAMS = {"swe_hip_gaussian": "swe_hip_gaussian_main.f90", "swe_hip_model": "swe_hip_model_main.f90"}
 
 
 def _run(cmd, what, verbose):

@@ -35,6 +35,7 @@
 
 #include "../../include/ch_swe.h"
 #include "swe_sbp.hpp"
+#include "swe_error.hpp"
 #include "swe_topology.hpp"
 
 namespace {
@@ -900,6 +901,10 @@ int download_t(swe_state *st, double *u, double *v, double *h)
     return SWE_OK;
 }
 }  // namespace
+
+namespace swe {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace swe
 
 // ==========================================================================
 // C ABI

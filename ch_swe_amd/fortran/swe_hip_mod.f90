@@ -128,10 +128,15 @@ contains
     end subroutine check
 
     ! create_domain + the linear operator's constants (shallow_water_mod.f90 L70-93)
-    function create_domain_hip(N, order, precision, radius, grav, h_mean, omega, coriolis_const) result(domain)
+    ! axis: rotation axis (default the z axis); flags: SWE_FLAG_* bits of
+    ! include/ch_swe.h (default 0)
+    function create_domain_hip(N, order, precision, radius, grav, h_mean, omega, coriolis_const, axis, flags) &
+            result(domain)
         integer,      intent(in) :: N, order, precision
         real(kind=8), intent(in) :: radius, grav, h_mean, omega
         logical,      intent(in) :: coriolis_const
+        real(kind=8), intent(in), optional :: axis(3)
+        integer,      intent(in), optional :: flags
         type(domain_hip_t)       :: domain
         type(swe_config_t)       :: cfg
         cfg%N = N
@@ -143,7 +148,9 @@ contains
         cfg%h_mean = h_mean
         cfg%omega = omega
         cfg%axis = [0.0_8, 0.0_8, 1.0_8]
+        if (present(axis)) cfg%axis = axis
         cfg%flags = 0
+        if (present(flags)) cfg%flags = flags
         cfg%block_threads = 0
         call check(swe_plan_create(cfg, domain%plan), "create_domain_hip")
         domain%N = N
