@@ -621,7 +621,7 @@ struct Engine : EngineBase {
     template <int O>
     void launch_records(const T *y, T *r)
     {
-        timer.launch(k_records<T, O, TY>, nseg, nthr, pl->stream, 1, d, cf, y, r);
+        timer.launch(k_records<T, O, TY>, nseg, F_REC_THREADS, pl->stream, 1, d, cf, y, r);
     }
 
     template <int O>
@@ -1206,6 +1206,14 @@ int check_tables_ty(int N)
         const int4 a = segment_of<TY>(N, i), b = ft.segs[i];
         if (a.x != b.x || a.y != b.y || a.z != b.z || a.w != b.w)
             return fail(SWE_EINVAL, "edge segment decode disagrees with the segment table");
+    }
+    // every segment's band fits the records kernels' frame
+    for (int i = 0; i < ft.nseg; i++) {
+        const int4 g = ft.segs[i];
+        RTile<float> rt;
+        rt.setup(N, g.x, g.y, g.z, g.w, nullptr);
+        if (rt.frame_points() > RSmem<float>::CAP || rt.frame_points() > F_CH * F_REC_THREADS)
+            return fail(SWE_EINVAL, "an edge segment's band exceeds the records frame");
     }
     // the launch order is a bijection of the tiles, side tiles first
     const int nt = ft.ntiles, e = side_tiles(ft.ntx, ft.nty);

@@ -412,7 +412,7 @@ struct Plan {
     template <int O>
     void records(const T *y, T *r)
     {
-        emu::launch(ft.nseg, F_THREADS, [&] { k_records<T, O, TY>(d, cf, y, r); });
+        emu::launch(ft.nseg, F_REC_THREADS, [&] { k_records<T, O, TY>(d, cf, y, r); });
     }
 
     template <int O, int MODE>
@@ -622,7 +622,7 @@ int run_multi(int N, int N2, int steps, unsigned seed)
     }
     const StageArgs<T> *A0 = args.data();
     const int2 *W = wl.data(), *R = rl.data();
-    emu::launch(rpre[2], F_THREADS, [&] { k_records_multi<T, O, TY, 0>(A0, R); });
+    emu::launch(rpre[2], F_REC_THREADS, [&] { k_records_multi<T, O, TY, 0>(A0, R); });
     for (int it = 0; it < ns[0]; it++) {
         const int k = it < ns[1] ? 2 : 1;  // members still running
         const int nw = wpre[k], nr = rpre[k];
@@ -631,8 +631,8 @@ int run_multi(int N, int N2, int steps, unsigned seed)
             if (s == 0) emu::launch(nw, F_THREADS, [&] { k_stage_multi<T, O, TY, 1>(a, W, nw); });
             else if (s == 3) emu::launch(nw, F_THREADS, [&] { k_stage_multi<T, O, TY, 3>(a, W, nw); });
             else emu::launch(nw, F_THREADS, [&] { k_stage_multi<T, O, TY, 2>(a, W, nw); });
-            if (s < 3) emu::launch(nr, F_THREADS, [&] { k_records_multi<T, O, TY, 1>(a, R); });
-            else emu::launch(nr, F_THREADS, [&] { k_records_multi<T, O, TY, 2>(a, R); });
+            if (s < 3) emu::launch(nr, F_REC_THREADS, [&] { k_records_multi<T, O, TY, 1>(a, R); });
+            else emu::launch(nr, F_REC_THREADS, [&] { k_records_multi<T, O, TY, 2>(a, R); });
         }
     }
     int bad = 0;
