@@ -68,7 +68,8 @@ class _CConfig(ctypes.Structure):
                 ("coriolis_const", ctypes.c_int), ("radius", ctypes.c_double),
                 ("grav", ctypes.c_double), ("h_mean", ctypes.c_double),
                 ("omega", ctypes.c_double), ("axis", ctypes.c_double * 3),
-                ("flags", ctypes.c_int), ("block_threads", ctypes.c_int)]
+                ("flags", ctypes.c_int), ("block_threads", ctypes.c_int),
+                ("tile_x0", ctypes.c_int), ("tile_y0", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -157,8 +158,9 @@ class Config:
     def __init__(self, N, order=42, precision=FP64, radius=6.371e6, grav=9.81,
                  h_mean=1000.0, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
                  coriolis_const=False, unfused=False, linear_blocks=False, block_threads=0,
-                 general_tiles=False, compensated=False):
+                 general_tiles=False, compensated=False, tile_x0=0, tile_y0=0):
         self.N, self.order, self.precision = int(N), int(order), int(precision)
+        self.tile_x0, self.tile_y0 = int(tile_x0), int(tile_y0)
         self.compensated = bool(compensated)
         self.general_tiles = bool(general_tiles)
         self.block_threads = int(block_threads)
@@ -177,6 +179,7 @@ class Config:
         c.flags = ((FLAG_UNFUSED if self.unfused else 0) | (FLAG_LINEAR_BLOCKS if self.linear_blocks else 0) |
                    (FLAG_GENERAL_TILES if self.general_tiles else 0) | (FLAG_COMPENSATED if self.compensated else 0))
         c.block_threads = self.block_threads
+        c.tile_x0, c.tile_y0 = self.tile_x0, self.tile_y0
         return c
 
 
@@ -290,17 +293,27 @@ class Rk4:
 def create_linear_swm_operator(N, order=42, precision=FP64, grav=9.81, h_mean=1000.0,
                                radius=6.371e6, omega=7.292e-5, axis=(0.0, 0.0, 1.0),
                                coriolis_const=False, unfused=False, linear_blocks=False, block_threads=0,
-                               general_tiles=False, compensated=False):
+                               general_tiles=False, compensated=False, tile_x0=0, tile_y0=0, tuned=True):
     """Factory mirroring create_linear_swm_operator
     (operator_swm_factory_mod.f90 L586-685) for the Ch staggering.
     unfused=True selects the multi-kernel diagnostic path; linear_blocks=True
     the plain block order of the fused stage kernel; general_tiles=True sends
     every tile through its general path; block_threads (0 or 512) its
     workgroup size; compensated=True runs RK4 with compensated summation of
-    each step's state update (states carry a compensation field)."""
+    each step's state update (states carry a compensation field); tile_x0 /
+    tile_y0 the first-tile widths of the fused stage kernel's tiling (0: the
+    library's split).  tuned=True takes the tiling and block order the
+    autotuner stored for (N, order, precision) (ch_swe_amd/autotune.py) when
+    none is given; results depend on them only at the level of rounding."""
+    if tuned and not unfused and not linear_blocks and tile_x0 == 0 and tile_y0 == 0:
+        from . import autotune
+        entry = autotune.lookup(N, order, precision)
+        if entry:
+            tile_x0, tile_y0 = entry["tile_x0"], entry["tile_y0"]
+            linear_blocks = bool(entry["linear_blocks"])
     return OperatorSwmLin(Config(N, order, precision, radius, grav, h_mean, omega, axis,
                                  coriolis_const, unfused, linear_blocks, block_threads, general_tiles,
-                                 compensated))
+                                 compensated, tile_x0, tile_y0))
 
 
 class Ensemble:

@@ -32,11 +32,51 @@ inline void tile_axis(int n, int tw, int &nt, int &fw)
     fw = m / 2;
 }
 
-inline int build_fused_tables(int N, int TY, FusedTables &ft, std::string &err)
+// A requested first-tile width fw_req (0: the split above) of an axis with
+// edge tiles: between 8 and m - 9, m = the points the full tiles leave.
+inline bool tile_axis_req(int n, int tw, int fw_req, int &nt, int &fw, const char *axis, std::string &err)
+{
+    tile_axis(n, tw, nt, fw);
+    if (fw_req == 0) return true;
+    const int k = (n - 17) / tw;
+    if (n < 17 + tw || k < 1) {
+        err = std::string("tile_") + axis + "0: this axis is tiled uniformly (too few points for edge tiles)";
+        return false;
+    }
+    const int m = n - k * tw;
+    if (fw_req < 8 || fw_req > m - 9) {
+        err = std::string("tile_") + axis + "0 = " + std::to_string(fw_req) + " out of range 8 .. " + std::to_string(m - 9);
+        return false;
+    }
+    fw = fw_req;
+    return true;
+}
+
+// Every tile of a tiling fits the stage kernel: at most F_THREADS points
+// (phase D takes one per thread) and its region 0 within the LDS frame
+// (F_TX + 17 by TY + 17 points).
+inline bool tiling_fits(int n, int TY, const FusedTables &ft, std::string &err)
+{
+    for (int ty = 0; ty < ft.nty; ty++)
+        for (int tx = 0; tx < ft.ntx; tx++) {
+            const Rg sx = tile_span(tx, ft.ntx, ft.fwx, F_TX, n), sy = tile_span(ty, ft.nty, ft.fwy, TY, n);
+            const int wx = sx.hi - sx.lo + 1, wy = sy.hi - sy.lo + 1;
+            const Rg rx = ext_rg(sx.lo - 6, sx.hi + 6, n, 14), ry = ext_rg(sy.lo - 6, sy.hi + 6, n, 14);
+            if (wx * wy > F_THREADS || rx.hi - rx.lo + 1 > F_TX + 17 || ry.hi - ry.lo + 1 > TY + 17) {
+                err = "tile of " + std::to_string(wx) + " x " + std::to_string(wy) +
+                      " points exceeds the stage kernel's limits (" + std::to_string(F_THREADS) + " points, LDS frame " +
+                      std::to_string(F_TX + 17) + " x " + std::to_string(TY + 17) + ")";
+                return false;
+            }
+        }
+    return true;
+}
+
+inline int build_fused_tables(int N, int TY, FusedTables &ft, std::string &err, int req_x = 0, int req_y = 0)
 {
     const int n = N + 1;
-    tile_axis(n, F_TX, ft.ntx, ft.fwx);
-    tile_axis(n, TY, ft.nty, ft.fwy);
+    if (!tile_axis_req(n, F_TX, req_x, ft.ntx, ft.fwx, "x", err) || !tile_axis_req(n, TY, req_y, ft.nty, ft.fwy, "y", err))
+        return 1;
     ft.ntiles = 6 * ft.ntx * ft.nty;
     ft.segs.clear();
     for (int p = 0; p < 6; p++)
@@ -51,6 +91,5 @@ inline int build_fused_tables(int N, int TY, FusedTables &ft, std::string &err)
             }
         }
     ft.nseg = (int)ft.segs.size();
-    (void)err;
-    return 0;
+    return tiling_fits(n, TY, ft, err) ? 0 : 1;
 }

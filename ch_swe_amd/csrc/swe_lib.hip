@@ -580,7 +580,7 @@ struct Engine : EngineBase {
         // tiles, edge segments and which tiles write into each segment's band
         FusedTables ft;
         std::string err;
-        if (build_fused_tables(N, TY, ft, err)) return fail(SWE_EINVAL, err);
+        if (build_fused_tables(N, TY, ft, err, pl->cfg.tile_x0, pl->cfg.tile_y0)) return fail(SWE_EINVAL, err);
         ntx = ft.ntx;
         nty = ft.nty;
         fwx = ft.fwx;
@@ -843,6 +843,13 @@ int validate_config(const swe_config *cfg, const char *who)
         return fail(SWE_EINVAL, w + "unknown bits in flags");
     if ((cfg->flags & SWE_FLAG_COMPENSATED) && (cfg->flags & SWE_FLAG_UNFUSED))
         return fail(SWE_EINVAL, w + "compensated summation runs on the fused path (not with SWE_FLAG_UNFUSED)");
+    if (cfg->tile_x0 || cfg->tile_y0) {
+        if (cfg->flags & SWE_FLAG_UNFUSED) return fail(SWE_EINVAL, w + "tile_x0 / tile_y0 apply to the fused path");
+        FusedTables ft;
+        std::string err;
+        if (build_fused_tables(cfg->N, cfg->precision == SWE_FP32 ? 16 : 8, ft, err, cfg->tile_x0, cfg->tile_y0))
+            return fail(SWE_EINVAL, w + err);
+    }
     if (!(cfg->radius > 0) || !std::isfinite(cfg->radius) || !std::isfinite(cfg->grav) ||
         !std::isfinite(cfg->h_mean) || !std::isfinite(cfg->omega) || !std::isfinite(cfg->axis[0]) ||
         !std::isfinite(cfg->axis[1]) || !std::isfinite(cfg->axis[2]))
@@ -1202,6 +1209,20 @@ int check_tables_ty(int N)
     FusedTables ft;
     std::string err;
     if (build_fused_tables(N, TY, ft, err)) return fail(SWE_EINVAL, err);
+    // every first-tile width a plan may request gives a tiling that fits,
+    // and one more or less is refused
+    for (int ax = 0; ax < 2; ax++) {
+        const int tw = ax == 0 ? F_TX : TY, n = N + 1, k = (n - 17) / tw;
+        if (n < 17 + tw || k < 1) continue;
+        const int m = n - k * tw;
+        for (int req = 7; req <= m - 8; req++) {
+            FusedTables f2;
+            const bool ok = build_fused_tables(N, TY, f2, err, ax == 0 ? req : 0, ax == 1 ? req : 0) == 0;
+            const bool in_range = req >= 8 && req <= m - 9;
+            if (!in_range && ok) return fail(SWE_EINVAL, "a first-tile width out of range was accepted");
+            if (ok && (ax == 0 ? f2.fwx : f2.fwy) != req) return fail(SWE_EINVAL, "a requested first-tile width was not applied");
+        }
+    }
     for (int i = 0; i < ft.nseg; i++) {
         const int4 a = segment_of<TY>(N, i), b = ft.segs[i];
         if (a.x != b.x || a.y != b.y || a.z != b.z || a.w != b.w)

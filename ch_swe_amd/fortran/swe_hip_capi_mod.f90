@@ -22,6 +22,8 @@ module swe_hip_capi_mod
         real(c_double) :: axis(3)
         integer(c_int) :: flags
         integer(c_int) :: block_threads
+        integer(c_int) :: tile_x0
+        integer(c_int) :: tile_y0
     end type swe_config_t
 
     public :: swe_plan_create, swe_plan_destroy, swe_state_create, swe_state_destroy, &

@@ -152,6 +152,8 @@ contains
         cfg%flags = 0
         if (present(flags)) cfg%flags = flags
         cfg%block_threads = 0
+        cfg%tile_x0 = 0
+        cfg%tile_y0 = 0
         call check(swe_plan_create(cfg, domain%plan), "create_domain_hip")
         domain%N = N
         domain%order = order

@@ -67,6 +67,9 @@ typedef struct {
     double axis[3];     /* rotation axis */
     int flags;          /* SWE_FLAG_* bits */
     int block_threads;  /* workgroup size of the fused kernels: 0 or 512 (both mean 512) */
+    int tile_x0;        /* width of the first tile along x / y of the fused stage kernel's tiling: */
+    int tile_y0;        /*   0 = the library's split; else 8 .. (what the axis leaves) - 9, within the
+                             tile limits (ch_swe_amd/autotune.py sweeps these) */
 } swe_config;
 
 const char *swe_last_error(void);

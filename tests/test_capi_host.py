@@ -55,3 +55,31 @@ def test_fused_tables_selfcheck(precision):
     edge-aligned tilings (no GPU needed)."""
     for N in list(range(4, 130)) + [191, 192, 255, 256, 383, 384, 511, 512, 1023]:
         cs.check_tables(N, precision)
+
+
+def test_tile_split_requests_are_validated():
+    """swe_config.tile_x0 / tile_y0 outside what the axis leaves are refused
+    before any device work, with the admissible range."""
+    with pytest.raises(cs.SweError, match=r"tile_x0 = 30 out of range 8 \.\. 24"):
+        cs.create_linear_swm_operator(384, 42, cs.FP32, tile_x0=30, tuned=False)
+    with pytest.raises(cs.SweError, match="tile_y0 = 9 out of range 8 .. 8"):
+        cs.create_linear_swm_operator(384, 42, cs.FP32, tile_y0=9, tuned=False)
+    with pytest.raises(cs.SweError, match="tiled uniformly"):
+        cs.create_linear_swm_operator(16, 42, cs.FP32, tile_x0=8, tuned=False)
+    with pytest.raises(cs.SweError, match="fused path"):
+        cs.create_linear_swm_operator(384, 42, cs.FP32, tile_x0=12, unfused=True, tuned=False)
+
+
+def test_autotune_choices_and_table(tmp_path, monkeypatch):
+    """The tuner's candidate widths are the ones the library accepts, and the
+    factory looks the tuned entry up by (N, order, precision)."""
+    from ch_swe_amd import autotune as at
+    assert at.axis_choices(384, 32) == [0] + list(range(8, 25))
+    assert at.axis_choices(384, 16) == [0, 8]
+    assert at.axis_choices(16, 32) == [0]
+    p = tmp_path / "tuning.json"
+    p.write_text('{"96/42/fp32": {"tile_x0": 12, "tile_y0": 0, "linear_blocks": false}}')
+    monkeypatch.setenv("CH_SWE_TUNING", str(p))
+    assert at.lookup(96, 42, cs.FP32)["tile_x0"] == 12
+    assert at.lookup(96, 42, cs.FP64) is None
+    assert at.key(384, 21, cs.FP64) == "384/21/fp64"

@@ -343,3 +343,27 @@ def test_compensated_state_semantics():
     rk4.step(a, op, dt, 30)
     for g, r in zip(a.download(), c.download()):
         assert np.array_equal(g, r)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [cs.FP64, cs.FP32])
+@pytest.mark.parametrize("N,x0,y0,linear", [(96, 8, 0, False), (96, 24, 0, True), (100, 13, 9, False),
+                                            (384, 20, 0, False)])
+def test_tile_splits_agree(precision, N, x0, y0, linear):
+    """Any first-tile widths a plan accepts, and either block order (the
+    autotuner's choices), give the library's split's RK4 steps to rounding
+    (a point moves between the inner and the edge path, which round
+    differently): 1e-12 relative in FP64, 1e-5 in FP32."""
+    _say(f"tile split N={N} x0={x0} y0={y0} linear={linear} precision={precision}")
+    rng = np.random.default_rng(N)
+    state = (rng.standard_normal((6, N + 1, N)), rng.standard_normal((6, N, N + 1)),
+             rng.standard_normal((6, N + 1, N + 1)))
+    out = []
+    for kw in ({}, dict(tile_x0=x0, tile_y0=y0, linear_blocks=linear)):
+        op = _op(N, 42, precision, tuned=False, **kw)
+        st = op.create_stvec().upload(*state)
+        cs.create_timescheme("rk4").step(st, op, 30.0, 3)
+        out.append(st.download())
+    tol = 1e-12 if precision == cs.FP64 else 1e-5
+    for a, b in zip(*out):
+        assert _rel(a, b) < tol
