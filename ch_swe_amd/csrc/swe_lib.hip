@@ -1233,7 +1233,7 @@ int check_tables_ty(int N)
         const int4 g = ft.segs[i];
         RTile<float> rt;
         rt.setup(N, g.x, g.y, g.z, g.w, nullptr);
-        if (rt.frame_points() > RSmem<float>::CAP || rt.frame_points() > F_CH * F_REC_THREADS)
+        if (rt.frame_points() > RSmem<float>::CAP || rt.frame_points() > RTile<float>::CH * F_REC_THREADS)
             return fail(SWE_EINVAL, "an edge segment's band exceeds the records frame");
     }
     // the launch order is a bijection of the tiles, side tiles first
