@@ -93,9 +93,10 @@ __device__ inline double sbp_weight(int idx, int n, bool centre, int order)
 // SBP quadrature weight J hx hy Ax(i) Ay(j) of point (i, j) on mesh m (0 xy,
 // 1 u at x centres, 2 v at y centres), FP64 (quadrature_factory_mod.f90
 // L89-188: create_C_sbp_quadrature)
+// mesh m: 0 xy, 1 u (x centres), 2 v (y centres), 3 o (cell centres)
 __device__ inline double quad_weight(int N, int order, double hx, int m, int i, int j)
 {
-    const int xt = m == 1, yt = m == 2;
+    const int xt = m == 1 || m == 3, yt = m == 2 || m == 3;
     const int nx = xt ? N : N + 1, ny = yt ? N : N + 1;
     const double al = -0.25 * PI + (i - 1 + 0.5 * xt) * hx;
     const double be = -0.25 * PI + (j - 1 + 0.5 * yt) * hx;

@@ -386,6 +386,8 @@ struct EngineBase {
     // part: >= 3 INT_MAX_BLOCKS doubles of partial sums, red: 4 doubles (device)
     virtual int integrals(void *st, double *part, double *red, double *mass, double *energy) = 0;
     virtual int integral_buffers(double **part, double **red) = 0;
+    // print_integrals_diag's values: hmin, hmax, mass, TE, KE, PE, enstrophy
+    virtual int integrals_diag(void *st, double grav, double out[7]) = 0;
     virtual int rk4(void *v0, void *acc, void *ya, void *yb, double dt, int nsteps) = 0;
     virtual bool fused() const = 0;
     virtual size_t elem() const = 0;
@@ -511,7 +513,9 @@ struct Engine : EngineBase {
         use_fused = !(p->cfg.flags & SWE_FLAG_UNFUSED);
         nthr = p->cfg.block_threads ? p->cfg.block_threads : F_THREADS;
         if ((rc = alloc((void **)&part, 3 * INT_MAX_BLOCKS * sizeof(double))) ||
-            (rc = alloc((void **)&red, 4 * sizeof(double))))
+            (rc = alloc((void **)&red, 4 * sizeof(double))) ||
+            (rc = alloc((void **)&dpart, (size_t)DIAG_K * INT_MAX_BLOCKS * sizeof(double))) ||
+            (rc = alloc((void **)&dout, 8 * sizeof(double))))
             return rc;
         if (use_fused) rc = init_fused(geo);
         else rc = ensure_full();
@@ -767,6 +771,25 @@ struct Engine : EngineBase {
         k_axpy<T><<<blocks_for(n), TPB, 0, pl->stream>>>(n, (T *)out, T(s1), (const T *)x, T(s2), (const T *)y,
                                                           update);
         HIP_TRY(hipGetLastError());
+        return SWE_OK;
+    }
+
+    double *dpart = nullptr, *dout = nullptr;  // diagnostics: partial results, totals
+    template <int O>
+    void launch_diag(const T *st, double grav)
+    {
+        const int grid = std::min(ntiles, INT_MAX_BLOCKS);
+        k_diag<T, O, TY><<<grid, F_THREADS, 0, pl->stream>>>(d, cf, geo, st, ntx, nty, fwx, fwy, grav, dpart);
+        k_diag_final<<<1, 64, 0, pl->stream>>>(dpart, grid, dout);
+    }
+    int integrals_diag(void *st, double grav, double out[7]) override
+    {
+        if (!use_fused) return fail(SWE_EINVAL, "swe_integrals_diag: diagnostics run on fused plans only");
+        if (order == 21) launch_diag<21>((const T *)st, grav);
+        else launch_diag<42>((const T *)st, grav);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, dout, 7 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+        HIP_TRY(hipStreamSynchronize(pl->stream));
         return SWE_OK;
     }
 
@@ -1123,6 +1146,15 @@ int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy)
     double *part = nullptr, *red = nullptr;
     if ((rc = plan->eng->integral_buffers(&part, &red))) return rc;
     return plan->eng->integrals(st->buf, part, red, mass, energy);
+}
+
+int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7])
+{
+    if (!plan || !out) return fail(SWE_EINVAL, "swe_integrals_diag: NULL argument");
+    if (!std::isfinite(grav)) return fail(SWE_EINVAL, "swe_integrals_diag: non-finite gravity");
+    int rc = check_state(plan, st, "state");
+    if (rc) return rc;
+    return plan->eng->integrals_diag(st->buf, grav, out);
 }
 
 }  // extern "C"

@@ -105,6 +105,7 @@ module swe_hip_mod
     contains
         procedure :: apply => operator_apply
         procedure :: integrals => operator_integrals
+        procedure :: integrals_diag => operator_integrals_diag
     end type operator_swm_lin_hip_t
 
     type, extends(timescheme_t) :: rk4_hip_t
@@ -267,6 +268,17 @@ contains
         real(kind=8),                  intent(out)   :: mass, energy
         call check(swe_integrals(domain%plan, hip_state(v), mass, energy), "integrals")
     end subroutine operator_integrals
+
+    ! swm_output_diag_mod.f90 print_integrals_diag L156-184: diag = hmin, hmax,
+    ! mass, TE, KE, PE, enstrophy (grav: the PE's gravity, Earth_grav there)
+    subroutine operator_integrals_diag(this, v, domain, grav, diag)
+        class(operator_swm_lin_hip_t), intent(inout) :: this
+        class(stvec_t),                intent(in)    :: v
+        type(domain_hip_t),            intent(in)    :: domain
+        real(kind=8),                  intent(in)    :: grav
+        real(kind=8),                  intent(out)   :: diag(7)
+        call check(swe_integrals_diag(domain%plan, hip_state(v), grav, diag), "integrals_diag")
+    end subroutine operator_integrals_diag
 
     ! rk4_mod.f90 step_rk4 L20-42 (fused stages on the device)
     subroutine rk4_step(this, v0, operator, domain, dt)

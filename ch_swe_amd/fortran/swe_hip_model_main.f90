@@ -29,11 +29,11 @@
 ! diagnostics step (Gaussian hill: against the reference solution file
 ! refsol_path when it exists, gaussian_hill_linear_test_mod.f90 L71-129;
 ! TS2: relative l2 / linf error of h, Williamson_test2_mod.f90 L115-143),
-! 'rec n' at every write step, and at every diagnostics step a line
-! hmin / hmax / mass / energy: h's extrema and quadrature mass as
-! print_integrals_diag (swm_output_diag_mod.f90 L156-184) prints them, and the
-! linear invariant energy of the operator (its nonlinear TE/KE/PE/enstrophy
-! and the lat-lon h.dat fields are not produced by this driver).
+! 'rec n' at every write step, and at every diagnostics step the line of
+! print_integrals_diag (swm_output_diag_mod.f90 L156-184): hmin, hmax, mass,
+! TE, KE, PE, enstrophy, computed on the device (swe_integrals_diag) with the
+! Ch operators of the paper's output_diag section (the lat-lon h.dat fields
+! are not produced by this driver).
 ! The final state is written to out.bin (default swe_hip_state.bin) in the
 ! packed (u, v, h) layout of include/ch_swe.h.
 program swe_hip_model
@@ -64,7 +64,7 @@ program swe_hip_model
     integer :: N, order, precision, flags, nstep_write, nstep_diagnostics, nsteps, it, nrun, p, i, j, k
     real(kind=8) :: dt, tau_write, tau_diagnostics, simulation_time, radius, omega, h_mean, axis(3), rm(9)
     real(kind=8) :: lam0, phi0, d0, u0, tc_h_mean, an(3), c(3), r(3), al, be, X, Y, s, vel(3), a1(3), a2(3)
-    real(kind=8) :: mass, energy, l2_h0, linf_h0
+    real(kind=8) :: diag(7), l2_h0, linf_h0
     logical :: coriolis_const, linear_mode, gauss, have_refsol
 
     call get_command_argument(1, cfg_file)
@@ -244,10 +244,14 @@ program swe_hip_model
         call timescheme%steps(state, domain, dt, nrun)
         it = it + nrun
         if (mod(it, nstep_diagnostics) == 0) then
-            call state%download(u, v, h)
-            call operator%integrals(state, domain, mass, energy)
-            print '(4(A,E24.16))', "hmin = ", minval(h), " hmax = ", maxval(h), " mass = ", mass, " energy = ", energy
-            call print_scores(it)
+            ! print_integrals_diag (swm_output_diag_mod.f90 L156-184), then the scores
+            call operator%integrals_diag(state, domain, Earth_grav, diag)
+            print '(7(A,E24.16))', "hmin = ", diag(1), " hmax = ", diag(2), " mass = ", diag(3), &
+                " TE = ", diag(4), " KE = ", diag(5), " PE = ", diag(6), " Enstrophy = ", diag(7)
+            if (.not. gauss .or. have_refsol) then
+                call state%download(u, v, h)
+                call print_scores(it)
+            end if
         end if
         if (mod(it, nstep_write) == 0) print *, "rec", it / nstep_write + 1
     end do

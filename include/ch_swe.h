@@ -123,6 +123,17 @@ int swe_integrals(swe_plan *plan, swe_state *st, double *mass, double *energy);
  * array and race: serialise them (one host thread, or one stream). */
 int swe_integrals_noalloc(swe_plan *plan, swe_state *st, double *mass, double *energy);
 
+/* The integral diagnostics the reference prints every diagnostics step:
+ * swm_output_diag_t%print_integrals_diag + calc_energies
+ * (models/shallow_water/output_and_diag/swm_output_diag_mod.f90 L156-234)
+ * with the Ch operators of the paper's output_diag section
+ * (interp2d_p2uv_Ch_sbpXX, interp2d_uv2qvec_Ch_sbpXX, curl_ch_sbp_sat_XX,
+ * co2contra_ch_sbpXX, SBP_CXX quadratures), covariant winds, no orography.
+ * out[7] = hmin, hmax, mass, TE, KE, PE, enstrophy (FP64 sums on the device);
+ * grav is the PE's gravity (the reference uses Earth_grav there whatever the
+ * operator's).  Fused plans only; blocks until the values are on the host. */
+int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7]);
+
 /* Serving: continuous batching of independent solves of mixed size and
  * precision under a device-memory budget (no counterpart in the reference,
  * which runs one solve per program).  Requests are queued with a copy of

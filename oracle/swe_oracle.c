@@ -248,6 +248,33 @@ static void build_ch_map(hmap *m, int N, int depth)
         }
 }
 
+/* tangential covariant components at the duplicated edge points: u (mesh y)
+ * across bottom/top, v (mesh x) across left/right, depth 1 (the neighbour's
+ * duplicate of the edge point) -- what the i2c SAT of curl_ch_sbp_sat reads
+ * from its Ch vector exchange (curl_sbp_sat_mod.f90 L26-44,
+ * sbp_diff_i2c_42_mod.f90 add_SAT_correction L162-274: f(0), f(n+2)). */
+static void build_tan_map(hmap *m, int N)
+{
+    for (int p = 0; p < 6; p++)
+        for (int side = 0; side < 4; side++) {
+            mtype mt = side < 2 ? MT_V : MT_U;
+            int c = side < 2 ? 1 : 0;
+            for (int t = 1; t <= N; t++) {
+                int q, si, sj, M[2][2];
+                mtype smt;
+                unfold(N, p, side, 1, t, mt, &q, &si, &sj, M, &smt);
+                int ti, tj;
+                if (side == 0) { ti = 0; tj = t; }
+                else if (side == 1) { ti = N + 2; tj = t; }
+                else if (side == 2) { ti = t; tj = 0; }
+                else { ti = t; tj = N + 2; }
+                int sc = M[c][0] != 0 ? 0 : 1;
+                hentry e = {p, ti, tj, c, q, si, sj, sc, M[c][sc]};
+                hmap_push(m, e);
+            }
+        }
+}
+
 static void apply_map(const hmap *m, fld *a, fld *b)
 {
     fld *F[2] = {a, b};
@@ -431,7 +458,7 @@ struct swo {
     geom G[3];
     double *fJ2;
     double *qb, *qt, *ql, *qr;
-    hmap m_xy, m_xyv, m_ch;
+    hmap m_xy, m_xyv, m_ch, m_tan;
     fld h, u, v, gx, gy, ut, vt, w1, w2, up, vp, Ju, Jv, Dx, Dy, div;
     /* RK4 registers */
     double *k[4][3], *y[3];
@@ -603,6 +630,7 @@ swo *swo_create(int N, int order, double radius, double grav, double h_mean,
     build_xy_map(&s->m_xy, N, 1, 0);
     build_xy_map(&s->m_xyv, N, 1, 1);
     build_ch_map(&s->m_ch, N, order == 21 ? 2 : 3);
+    build_tan_map(&s->m_tan, N);
     fld_init(&s->h, N + 1, N + 1);
     fld_init(&s->u, N, N + 1);
     fld_init(&s->v, N + 1, N);
@@ -636,7 +664,7 @@ void swo_destroy(swo *s)
         free(G->rx); free(G->ry); free(G->rz); free(G->w);
     }
     free(s->fJ2); free(s->qb); free(s->qt); free(s->ql); free(s->qr);
-    free(s->m_xy.e); free(s->m_xyv.e); free(s->m_ch.e);
+    free(s->m_xy.e); free(s->m_xyv.e); free(s->m_ch.e); free(s->m_tan.e);
     fld *F[] = {&s->h, &s->u, &s->v, &s->gx, &s->gy, &s->ut, &s->vt, &s->w1, &s->w2,
                 &s->up, &s->vp, &s->Ju, &s->Jv, &s->Dx, &s->Dy, &s->div};
     for (size_t k = 0; k < sizeof(F) / sizeof(F[0]); k++) fld_free(F[k]);
@@ -1028,6 +1056,139 @@ double swo_energy(swo *s, const double *u, const double *v, const double *h)
     free(ut);
     free(vt);
     return 0.5 * s->g * eh + 0.5 * s->H0 * (eu + ev);
+}
+
+/* cell-centre (mesh o) geometry: J, the SBP_C quadrature weight and the
+ * Coriolis parameter (metric_2d_ecs_mod.f90 L338-411 at the o points,
+ * sbp_quadrature_mod.f90 L22-41 with the centre weights in both
+ * directions, coriolis_factory_mod.f90 calc_coriolis_parameter L239-280) */
+static void centre_geom(const swo *s, int p, int i, int j, double *J, double *w, double *f)
+{
+    const int N = s->N;
+    const double al = -0.25 * PI + (i - 0.5) * s->hx, be = -0.25 * PI + (j - 0.5) * s->hx;
+    const double X = tan(al), Y = tan(be), sg = sqrt(1 + X * X + Y * Y);
+    *J = (1 + X * X) * (1 + Y * Y) / (sg * sg * sg);
+    double Ac[4] = {1, 1, 1, 1};
+    int nc = 0;
+    if (s->order == 42) {
+        Ac[0] = 13.0 / 12; Ac[1] = 7.0 / 8; Ac[2] = 25.0 / 24; Ac[3] = 1;
+        nc = 4;
+    }
+    double wx = 1, wy = 1;
+    if (i <= nc) wx = Ac[i - 1];
+    if (N - i + 1 <= nc) wx = Ac[N - i];
+    if (j <= nc) wy = Ac[j - 1];
+    if (N - j + 1 <= nc) wy = Ac[N - j];
+    *w = *J * s->hx * s->hx * wx * wy;
+    if (s->cor_const) {
+        *f = s->omega;
+    } else {
+        double pr[3] = {X / sg, Y / sg, 1 / sg}, r[3];
+        proto2real(p, pr, r);
+        double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        *f = 2 * s->omega * (s->axis[0] * r[0] + s->axis[1] * r[1] + s->axis[2] * r[2]) / rn;
+    }
+}
+
+/* curl_sbp_sat_mod.f90 calc_curl L26-74 for Ch (curl_ch_sbp_sat_XX, mesh o):
+ * Dy = i2c derivative of u along y, Dx = of v along x, each with its SAT
+ * against the neighbour's tangential component at the duplicated edge
+ * point; curl = (Dx/hx - Dy/hy) / (J a) at the cell centres.  u, v loaded
+ * into s->u, s->v; the result into s->div (centres, (N, N) per panel). */
+static void op_curl(swo *s)
+{
+    const int N = s->N;
+    apply_map(&s->m_tan, &s->u, &s->v);
+    for (int p = 0; p < 6; p++) {
+        OMP_FOR
+        for (int j = 1; j <= N; j++)  /* Dx at (centre i, centre j): v row j */
+            d_i2c(s->order, AT(&s->v, p, 0, j), 1, N, AT(&s->Dx, p, 0, j), 1);
+        OMP_FOR
+        for (int i = 1; i <= N; i++)  /* Dy: u column i */
+            d_i2c(s->order, AT(&s->u, p, i, 0), s->u.sx, N, AT(&s->Dy, p, i, 0), s->Dy.sx);
+        OMP_FOR
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                double J, w, f;
+                centre_geom(s, p, i, j, &J, &w, &f);
+                *AT(&s->div, p, i, j) = (*AT(&s->Dx, p, i, j) / s->hx - *AT(&s->Dy, p, i, j) / s->hx) / (J * s->a);
+            }
+    }
+}
+
+void swo_curl(swo *s, const double *u, const double *v, double *curl)
+{
+    const int N = s->N;
+    fld_load(&s->u, u);
+    fld_load(&s->v, v);
+    op_curl(s);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) curl[((size_t)p * N + (j - 1)) * N + (i - 1)] = *AT(&s->div, p, i, j);
+}
+
+/* print_integrals_diag + calc_energies (swm_output_diag_mod.f90 L156-234)
+ * for the Ch staggering with covariant winds and no orography: out = hmin,
+ * hmax, mass, TE, KE, PE, enstrophy.  h2uv = interp2d_p2uv_Ch_sbpXX (i2c
+ * interpolation of h along x to u points, along y to v points); uv2q =
+ * interp2d_uv2qvec_Ch_sbpXX (i2c along y of hu, along x of hv, to the
+ * centres, averaged); the quadratures SBP_CXX of each mesh; grav as given
+ * (the reference passes Earth_grav). */
+void swo_integrals_diag(swo *s, const double *u, const double *v, const double *h, double grav, double out[7])
+{
+    const int N = s->N;
+    const int nh = swo_npts(s, SWO_MESH_XY);
+    double hmin = h[0], hmax = h[0], mass = 0, pe = 0, ke_u = 0, ke_v = 0, ens = 0;
+    for (int k = 0; k < nh; k++) {
+        hmin = h[k] < hmin ? h[k] : hmin;
+        hmax = h[k] > hmax ? h[k] : hmax;
+        mass += h[k] * s->G[SWO_MESH_XY].w[k];
+        pe += 0.5 * grav * h[k] * h[k] * s->G[SWO_MESH_XY].w[k];
+    }
+    fld_load(&s->u, u);
+    fld_load(&s->v, v);
+    fld_load(&s->h, h);
+    op_co2contra(s); /* s->ut, s->vt */
+    /* hu at u points (s->Ju), hv at v points (s->Jv) */
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N + 1; j++) i_i2c(s->order, AT(&s->h, p, 0, j), 1, N, AT(&s->Ju, p, 0, j), 1);
+        for (int i = 1; i <= N + 1; i++)
+            i_i2c(s->order, AT(&s->h, p, i, 0), s->h.sx, N, AT(&s->Jv, p, i, 0), s->Jv.sx);
+    }
+    const geom *Gu = &s->G[SWO_MESH_U], *Gv = &s->G[SWO_MESH_V];
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++)
+                ke_u += 0.5 * (*AT(&s->Ju, p, i, j) * *AT(&s->ut, p, i, j)) * *AT(&s->u, p, i, j) *
+                        Gu->w[PK(s, SWO_MESH_U, p, i, j)];
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++)
+                ke_v += 0.5 * (*AT(&s->Jv, p, i, j) * *AT(&s->vt, p, i, j)) * *AT(&s->v, p, i, j) *
+                        Gv->w[PK(s, SWO_MESH_V, p, i, j)];
+    }
+    /* hq at the centres: i2c along y of hu (s->w1), along x of hv (s->w2) */
+    for (int p = 0; p < 6; p++) {
+        for (int i = 1; i <= N; i++)
+            i_i2c(s->order, AT(&s->Ju, p, i, 0), s->Ju.sx, N, AT(&s->w1, p, i, 0), s->w1.sx);
+        for (int j = 1; j <= N; j++) i_i2c(s->order, AT(&s->Jv, p, 0, j), 1, N, AT(&s->w2, p, 0, j), 1);
+    }
+    op_curl(s); /* s->div */
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                double J, w, f;
+                centre_geom(s, p, i, j, &J, &w, &f);
+                const double hq = 0.5 * *AT(&s->w1, p, i, j) + 0.5 * *AT(&s->w2, p, i, j);
+                const double c = *AT(&s->div, p, i, j) + f;
+                ens += (0.5 * c * c) / hq * w;
+            }
+    out[0] = hmin;
+    out[1] = hmax;
+    out[2] = mass;
+    out[4] = ke_u + ke_v;
+    out[5] = pe;
+    out[3] = out[4] + out[5];
+    out[6] = ens;
 }
 
 /* test_fields_mod.f90 generate_gaussian_hill_scalar_field L491-514 (times amp) */

@@ -65,6 +65,10 @@ void swo_coriolis_fJ2(const swo *s, double *fJ2);
 void swo_sync_weights(const swo *s, double *qb, double *qt, double *ql, double *qr);
 double swo_energy(swo *s, const double *u, const double *v, const double *h);
 double swo_mass(const swo *s, const double *h);
+/* curl_ch_sbp_sat_XX at the cell centres ((N, N) per panel) */
+void swo_curl(swo *s, const double *u, const double *v, double *curl);
+/* print_integrals_diag: hmin, hmax, mass, TE, KE, PE, enstrophy */
+void swo_integrals_diag(swo *s, const double *u, const double *v, const double *h, double grav, double out[7]);
 void swo_gaussian_hill(const swo *s, double lam0, double phi0, double d0,
                        double amp, double *h);
 void swo_ts2_linear(const swo *s, double u0, double h_mean, double *u,

@@ -51,6 +51,8 @@ def _load():
     lib.swo_energy.restype = ctypes.c_double
     lib.swo_mass.argtypes = [ctypes.c_void_p, _dp]
     lib.swo_mass.restype = ctypes.c_double
+    lib.swo_curl.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    lib.swo_integrals_diag.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
     lib.swo_gaussian_hill.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]
     lib.swo_ts2_linear.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
     lib.swo_covariant_from_cart.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int, _dp]
@@ -196,6 +198,20 @@ class Oracle:
     def mass(self, h):
         h = self._c(h)
         return self.lib.swo_mass(self._h, _p(h))
+
+    def curl(self, u, v):
+        """curl_ch_sbp_sat at the cell centres, shape (6, N, N)."""
+        u, v = self._c(u), self._c(v)
+        out = np.zeros((6, self.N, self.N))
+        self.lib.swo_curl(self._h, _p(u), _p(v), _p(out))
+        return out
+
+    def integrals_diag(self, u, v, h, grav=9.80616):
+        """print_integrals_diag: dict of hmin, hmax, mass, TE, KE, PE, Enstrophy."""
+        u, v, h = self._c(u), self._c(v), self._c(h)
+        out = np.zeros(7)
+        self.lib.swo_integrals_diag(self._h, _p(u), _p(v), _p(h), grav, _p(out))
+        return dict(zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), out.tolist()))
 
     def gaussian_hill(self, lam0, phi0, d0, amp):
         h = self.zeros(MESH_XY)

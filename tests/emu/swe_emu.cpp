@@ -21,7 +21,7 @@
 // multi-instance launches of ensembles, the tile-wise integrals).
 //
 // usage: swe_emu <case> N order prec [steps] [seed] [flags]
-//   case  : apply | rk4 | integrals | multi
+//   case  : apply | rk4 | integrals | diag | multi
 //   prec  : 64 | 32
 //   flags : bit 0 every tile on the general path, bit 1 plain block order,
 //           bit 2 compensated summation (rk4)
@@ -327,10 +327,10 @@ struct Plan {
     int flags;
     std::vector<std::vector<T>> pool;
 
+    GeomParams g{};
     Plan(int N_, int order_, int flags_) : N(N_), order(order_), d(make_dims(N_)), flags(flags_)
     {
         const int n = N + 1;
-        GeomParams g{};
         g.hx = 0.5 * PI / N;
         g.radius = RADIUS;
         g.omega = OMEGA;
@@ -477,6 +477,18 @@ struct Plan {
         mass = s[0];
         energy = 0.5 * GRAV * s[1] + 0.5 * HMEAN * s[2];
     }
+
+    // Engine::integrals_diag
+    template <int O>
+    void diag(const T *st, double grav, double out[7])
+    {
+        const int grid = std::min(ft.ntiles, INT_MAX_BLOCKS);
+        std::vector<double> prt(DIAG_K * (size_t)grid, NAN), o(8, NAN);
+        double *pp = prt.data(), *op = o.data();
+        emu::launch(grid, F_THREADS, [&] { k_diag<T, O, TY>(d, cf, g, st, ft.ntx, ft.nty, ft.fwx, ft.fwy, grav, pp); });
+        emu::launch_flat(1, 64, [&] { k_diag_final(pp, grid, op); });
+        for (int k = 0; k < 7; k++) out[k] = o[k];
+    }
 };
 
 double rel(const std::vector<double> &a, const std::vector<double> &b)
@@ -560,6 +572,22 @@ int run_case(const std::string &cs, int N, int steps, unsigned seed, int flags)
         const double em = std::fabs(m - mr) / (hmax * 4 * PI), ee = std::fabs(e / er - 1);
         std::printf("integrals mass err %.3e energy rel err %.3e (bar %.1e)\n", em, ee, ibar);
         bad = !(em <= ibar) || !(ee <= ibar);
+    } else if (cs == "diag") {
+        T *st = pl.state();
+        // a state near rest on a deep layer (the enstrophy divides by h)
+        Fields s1 = s0;
+        for (double &x : s1.h) x = 500.0 + 10.0 * x;
+        pl.upload(st, s1.u.data(), s1.v.data(), s1.h.data());
+        double got[7], ref[7];
+        pl.template diag<O>(st, 9.80616, got);
+        swo_integrals_diag(o, s1.u.data(), s1.v.data(), s1.h.data(), 9.80616, ref);
+        const double dbar = sizeof(T) == 8 ? 1e-12 : 2e-6;
+        const char *nm[7] = {"hmin", "hmax", "mass", "TE", "KE", "PE", "enstrophy"};
+        for (int k = 0; k < 7; k++) {
+            const double e = std::fabs(got[k] - ref[k]) / std::max(std::fabs(ref[k]), 1e-300);
+            std::printf("diag %s %.17g oracle %.17g rel err %.3e (bar %.1e)\n", nm[k], got[k], ref[k], e, dbar);
+            bad |= !(e <= dbar);
+        }
     } else {
         std::fprintf(stderr, "unknown case %s\n", cs.c_str());
         bad = 2;
@@ -665,7 +693,7 @@ int dispatch(const std::string &cs, int N, int order, int steps, unsigned seed, 
 int main(int argc, char **argv)
 {
     if (argc < 5) {
-        std::fprintf(stderr, "usage: %s apply|rk4|integrals|multi N order 64|32 [steps] [seed] [flags]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s apply|rk4|integrals|diag|multi N order 64|32 [steps] [seed] [flags]\n", argv[0]);
         return 2;
     }
     const std::string cs = argv[1];

@@ -6,8 +6,8 @@ blocks in a random order, every device buffer an exact-size heap allocation
 (AddressSanitizer reports any access outside it) and the state padding NaN.
 The cases cover single-tile panels, uneven last tiles, interior tiles, the
 general-tiles and plain-order flags, both SBP orders and precisions, the RK4
-stage modes (compensated summation included), the multi-instance (ensemble) kernels and the tile-wise
-integrals.  No GPU is used."""
+stage modes (compensated summation included), the multi-instance (ensemble) kernels, the tile-wise
+integrals and the integral diagnostics.  No GPU is used."""
 import os
 import shutil
 import subprocess
@@ -34,6 +34,10 @@ CASES = [
     ("integrals", 48, 21, 32, 1, 10, 0),
     ("multi", 12, 42, 64, 2, 11, 0),    # two members (N, N + 7), own dt and steps
     ("multi", 9, 21, 32, 2, 12, 0),
+    ("diag", 20, 42, 64, 1, 15, 0),     # integral diagnostics (curl SAT halos, centre geometry)
+    ("diag", 41, 21, 32, 1, 16, 0),
+    ("diag", 57, 42, 64, 1, 17, 0),     # with interior tiles
+    ("diag", 32, 21, 64, 1, 18, 0),     # a one-point last tile along x: the centres before it read the SAT halo
 ]
 
 

@@ -58,11 +58,18 @@ def test_gauss_config_matches_oracle(oracle_mod, tmp_path, order, precision, tol
     ref = o.rk4(o.zeros(MESH_U), o.zeros(MESH_V), h0, 900.0, 8)
     for g, r in zip(_split(raw, N), ref):
         assert _rel(g, r) < tol
-    # the run loop's output: a diagnostics line per hour, 'rec' 2 and 3
-    assert len(re.findall(r"hmin = .* mass = .* energy = ", stdout)) == 2
+    # the run loop's output: print_integrals_diag's line per hour, 'rec' 2 and 3
+    lines = re.findall(r"hmin = (.*) hmax = (.*) mass = (.*) TE = (.*) KE = (.*) PE = (.*) Enstrophy = (.*)", stdout)
+    assert len(lines) == 2
     assert re.search(r"rec\s+2\b", stdout) and re.search(r"rec\s+3\b", stdout)
-    mass = [float(m) for m in re.findall(r"mass = \s*(\S+)", stdout)]
-    assert mass[0] == pytest.approx(o.mass(ref[2]), rel=1e-6)
+    # the last line is the final state's: against the oracle's diagnostics of
+    # its RK4 result (the enstrophy divides by h, a perturbation near zero
+    # here, so it is compared in the TS2 test)
+    d = o.integrals_diag(*ref, EARTH_GRAV)
+    vals = [float(x) for x in lines[-1]]
+    dtol = 1e-10 if precision.startswith("fp64") else 1e-4
+    for k, g in zip(("hmin", "hmax", "mass", "TE", "KE", "PE"), vals):
+        assert g == pytest.approx(d[k], rel=dtol, abs=dtol * abs(d["hmax"])), (k, g, d[k])
 
 
 @pytest.mark.gpu
@@ -83,6 +90,10 @@ def test_ts2_config_matches_oracle(oracle_mod, tmp_path, order):
         assert _rel(g, r) < 1e-12
         # and the initial state is the one the oracle builds (steady: small drift)
         assert _rel(g, i) < 1e-2
+    lines = re.findall(r"hmin = (.*) hmax = (.*) mass = (.*) TE = (.*) KE = (.*) PE = (.*) Enstrophy = (.*)", stdout)
+    d = o.integrals_diag(*ref, EARTH_GRAV)
+    for k, g in zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), (float(x) for x in lines[-1])):
+        assert g == pytest.approx(d[k], rel=1e-10), (k, g, d[k])
     errs = re.findall(r"Errors, t=\s*(\S+)\s+hours, l2_h =\s*(\S+)\s*, linf_h =\s*(\S+)", stdout)
     assert [float(e[0]) for e in errs] == [0.0, 1.0, 2.0]
     assert float(errs[0][1]) == 0.0 and 0 < float(errs[2][1]) < 1e-3

@@ -367,3 +367,29 @@ def test_tile_splits_agree(precision, N, x0, y0, linear):
     tol = 1e-12 if precision == cs.FP64 else 1e-5
     for a, b in zip(*out):
         assert _rel(a, b) < tol
+
+
+# N=32 FP64 and N=33 FP32: tilings that end one point short of a side (the
+# centres before it read the curl's SAT halo)
+DIAG_CASES = [(21, 20, cs.FP64, 0.0), (42, 48, cs.FP64, 7.292e-5), (42, 97, cs.FP64, 0.0), (21, 41, cs.FP32, 7.292e-5),
+              (42, 96, cs.FP32, 0.0), (21, 32, cs.FP64, 0.0), (42, 33, cs.FP32, 0.0)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,N,precision,omega", DIAG_CASES)
+def test_integrals_diag_matches_oracle(oracle_mod, order, N, precision, omega):
+    """swe_integrals_diag (print_integrals_diag's hmin, hmax, mass, TE, KE, PE,
+    enstrophy) against the oracle on a random state over a deep layer: FP64
+    1e-12 relative, FP32 2e-6; omega = 0 leaves the enstrophy to the curl
+    alone (its SAT halos included)."""
+    _say(f"integrals_diag order={order} N={N} precision={precision} omega={omega}")
+    o = oracle_mod.Oracle(N, order, radius=6371229.0, omega=omega)
+    u, v, h = _random_state(o, 40 + N)
+    h = 500.0 + 10.0 * h
+    op = _op(N, order, precision, radius=6371229.0, omega=omega)
+    st = op.create_stvec().upload(u, v, h)
+    got = op.integrals_diag(st, 9.80616)
+    ref = o.integrals_diag(u, v, h, 9.80616)
+    tol = 1e-12 if precision == cs.FP64 else 2e-6
+    for k, r in ref.items():
+        assert abs(got[k] - r) <= tol * abs(r), (k, got[k], r)
