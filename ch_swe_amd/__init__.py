@@ -38,7 +38,7 @@ EXPORTS = (
     "swe_last_error", "swe_version", "swe_plan_create", "swe_plan_destroy", "swe_plan_npts",
     "swe_plan_stream", "swe_synchronize", "swe_state_create", "swe_state_destroy",
     "swe_state_upload", "swe_state_download", "swe_apply", "swe_assign", "swe_update",
-    "swe_rk4_steps", "swe_integrals", "swe_integrals_noalloc", "swe_integrals_diag", "swe_halo_map", "swe_ensemble_create",
+    "swe_rk4_steps", "swe_integrals", "swe_integrals_noalloc", "swe_integrals_diag", "swe_integral_tends", "swe_halo_map", "swe_ensemble_create",
     "swe_ensemble_destroy", "swe_ensemble_rk4", "swe_ensemble_stream", "swe_ensemble_synchronize",
     "swe_plan_timing", "swe_plan_kernel_times", "swe_server_create", "swe_server_destroy",
     "swe_server_submit", "swe_server_tick", "swe_server_wait", "swe_server_take", "swe_server_stats",
@@ -106,6 +106,7 @@ def lib():
     L.swe_integrals.argtypes = [_vp, _vp, _dp, _dp]
     L.swe_integrals_noalloc.argtypes = [_vp, _vp, _dp, _dp]
     L.swe_integrals_diag.argtypes = [_vp, _vp, ctypes.c_double, _dp]
+    L.swe_integral_tends.argtypes = [_vp, _vp, ctypes.c_double, _dp]
     L.swe_halo_map.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip] + [_ip] * 9
     L.swe_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
     L.swe_ensemble_destroy.argtypes = [_vp]
@@ -270,6 +271,13 @@ class OperatorSwmLin:
         out = np.zeros(7)
         _check(lib().swe_integrals_diag(self.plan.h, state.h, float(grav), _p(out)))
         return dict(zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), out.tolist()))
+
+    def integral_tends(self, state, grav=9.80616):
+        """print_integral_tends (swm_output_diag_mod.f90 L236-342) of this
+        operator's tendency at state: dict of TE_tend, Enstrophy_tend."""
+        out = np.zeros(2)
+        _check(lib().swe_integral_tends(self.plan.h, state.h, float(grav), _p(out)))
+        return dict(zip(("TE_tend", "Enstrophy_tend"), out.tolist()))
 
     def synchronize(self):
         _check(lib().swe_synchronize(self.plan.h))

@@ -388,6 +388,8 @@ struct EngineBase {
     virtual int integral_buffers(double **part, double **red) = 0;
     // print_integrals_diag's values: hmin, hmax, mass, TE, KE, PE, enstrophy
     virtual int integrals_diag(void *st, double grav, double out[7]) = 0;
+    // print_integral_tends's values of the operator's tendency: TE_tend, Enstrophy_tend
+    virtual int integral_tends(void *st, double grav, double out[2]) = 0;
     virtual int rk4(void *v0, void *acc, void *ya, void *yb, double dt, int nsteps) = 0;
     virtual bool fused() const = 0;
     virtual size_t elem() const = 0;
@@ -793,6 +795,28 @@ struct Engine : EngineBase {
         return SWE_OK;
     }
 
+    T *tbuf = nullptr;  // the tendency of swe_integral_tends (allocated on its first call)
+    template <int O>
+    void launch_tends(const T *st, double grav)
+    {
+        const int grid = std::min(ntiles, INT_MAX_BLOCKS);
+        k_tends<T, O, TY><<<grid, F_THREADS, 0, pl->stream>>>(d, cf, geo, st, tbuf, ntx, nty, fwx, fwy, grav, dpart);
+        k_tends_final<<<1, 64, 0, pl->stream>>>(dpart, grid, dout);
+    }
+    int integral_tends(void *st, double grav, double out[2]) override
+    {
+        if (!use_fused) return fail(SWE_EINVAL, "swe_integral_tends: diagnostics run on fused plans only");
+        int rc;
+        if (!tbuf && (rc = alloc((void **)&tbuf, 3 * (size_t)d.F * sizeof(T)))) return rc;
+        if ((rc = apply(tbuf, st))) return rc;
+        if (order == 21) launch_tends<21>((const T *)st, grav);
+        else launch_tends<42>((const T *)st, grav);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, dout, 2 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        return SWE_OK;
+    }
+
     template <int O>
     void fused_integrals(const T *st, double *prt, double *rd)
     {
@@ -1155,6 +1179,15 @@ int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7]
     int rc = check_state(plan, st, "state");
     if (rc) return rc;
     return plan->eng->integrals_diag(st->buf, grav, out);
+}
+
+int swe_integral_tends(swe_plan *plan, swe_state *st, double grav, double out[2])
+{
+    if (!plan || !out) return fail(SWE_EINVAL, "swe_integral_tends: NULL argument");
+    if (!std::isfinite(grav)) return fail(SWE_EINVAL, "swe_integral_tends: non-finite gravity");
+    int rc = check_state(plan, st, "state");
+    if (rc) return rc;
+    return plan->eng->integral_tends(st->buf, grav, out);
 }
 
 }  // extern "C"

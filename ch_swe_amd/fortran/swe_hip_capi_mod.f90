@@ -28,7 +28,7 @@ module swe_hip_capi_mod
 
     public :: swe_plan_create, swe_plan_destroy, swe_state_create, swe_state_destroy, &
               swe_state_upload, swe_state_download, swe_apply, swe_assign, swe_update,  &
-              swe_rk4_steps, swe_integrals, swe_integrals_diag, swe_synchronize, swe_error_message
+              swe_rk4_steps, swe_integrals, swe_integrals_diag, swe_integral_tends, swe_synchronize, swe_error_message
 
     interface
         integer(c_int) function swe_plan_create(cfg, plan) bind(C, name="swe_plan_create")
@@ -93,6 +93,12 @@ module swe_hip_capi_mod
             type(c_ptr), value :: plan, st
             real(c_double), value :: grav
             real(c_double), intent(out) :: out(7)
+        end function
+        integer(c_int) function swe_integral_tends(plan, st, grav, out) bind(C, name="swe_integral_tends")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: plan, st
+            real(c_double), value :: grav
+            real(c_double), intent(out) :: out(2)
         end function
         type(c_ptr) function swe_last_error() bind(C, name="swe_last_error")
             import :: c_ptr

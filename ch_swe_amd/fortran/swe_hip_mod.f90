@@ -106,6 +106,7 @@ module swe_hip_mod
         procedure :: apply => operator_apply
         procedure :: integrals => operator_integrals
         procedure :: integrals_diag => operator_integrals_diag
+        procedure :: integral_tends => operator_integral_tends
     end type operator_swm_lin_hip_t
 
     type, extends(timescheme_t) :: rk4_hip_t
@@ -279,6 +280,17 @@ contains
         real(kind=8),                  intent(out)   :: diag(7)
         call check(swe_integrals_diag(domain%plan, hip_state(v), grav, diag), "integrals_diag")
     end subroutine operator_integrals_diag
+
+    ! swm_output_diag_mod.f90 print_integral_tends L236-342 of this operator:
+    ! tends = TE_tend, Enstrophy_tend
+    subroutine operator_integral_tends(this, v, domain, grav, tends)
+        class(operator_swm_lin_hip_t), intent(inout) :: this
+        class(stvec_t),                intent(in)    :: v
+        type(domain_hip_t),            intent(in)    :: domain
+        real(kind=8),                  intent(in)    :: grav
+        real(kind=8),                  intent(out)   :: tends(2)
+        call check(swe_integral_tends(domain%plan, hip_state(v), grav, tends), "integral_tends")
+    end subroutine operator_integral_tends
 
     ! rk4_mod.f90 step_rk4 L20-42 (fused stages on the device)
     subroutine rk4_step(this, v0, operator, domain, dt)

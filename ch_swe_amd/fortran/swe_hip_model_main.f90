@@ -32,8 +32,10 @@
 ! 'rec n' at every write step, and at every diagnostics step the line of
 ! print_integrals_diag (swm_output_diag_mod.f90 L156-184): hmin, hmax, mass,
 ! TE, KE, PE, enstrophy, computed on the device (swe_integrals_diag) with the
-! Ch operators of the paper's output_diag section (the lat-lon h.dat fields
-! are not produced by this driver).
+! Ch operators of the paper's output_diag section, followed by the line of
+! print_integral_tends (L236-342, "dynamics: TE_tend = ..., Enstrophy_tend =
+! ...", swe_integral_tends) (the lat-lon h.dat fields are not produced by this
+! driver).
 ! The final state is written to out.bin (default swe_hip_state.bin) in the
 ! packed (u, v, h) layout of include/ch_swe.h.
 program swe_hip_model
@@ -64,7 +66,7 @@ program swe_hip_model
     integer :: N, order, precision, flags, nstep_write, nstep_diagnostics, nsteps, it, nrun, p, i, j, k
     real(kind=8) :: dt, tau_write, tau_diagnostics, simulation_time, radius, omega, h_mean, axis(3), rm(9)
     real(kind=8) :: lam0, phi0, d0, u0, tc_h_mean, an(3), c(3), r(3), al, be, X, Y, s, vel(3), a1(3), a2(3)
-    real(kind=8) :: diag(7), l2_h0, linf_h0
+    real(kind=8) :: diag(7), tends(2), l2_h0, linf_h0
     logical :: coriolis_const, linear_mode, gauss, have_refsol
 
     call get_command_argument(1, cfg_file)
@@ -244,10 +246,14 @@ program swe_hip_model
         call timescheme%steps(state, domain, dt, nrun)
         it = it + nrun
         if (mod(it, nstep_diagnostics) == 0) then
-            ! print_integrals_diag (swm_output_diag_mod.f90 L156-184), then the scores
+            ! print_integrals_diag (swm_output_diag_mod.f90 L156-184) and
+            ! print_integral_tends (L236-342, shallow_water_mod.f90 L126-127),
+            ! then the scores
             call operator%integrals_diag(state, domain, Earth_grav, diag)
             print '(7(A,E24.16))', "hmin = ", diag(1), " hmax = ", diag(2), " mass = ", diag(3), &
                 " TE = ", diag(4), " KE = ", diag(5), " PE = ", diag(6), " Enstrophy = ", diag(7)
+            call operator%integral_tends(state, domain, Earth_grav, tends)
+            print '(2(A,E20.12))', "dynamics: TE_tend = ", tends(1), ", Enstrophy_tend = ", tends(2)
             if (.not. gauss .or. have_refsol) then
                 call state%download(u, v, h)
                 call print_scores(it)

@@ -134,6 +134,15 @@ int swe_integrals_noalloc(swe_plan *plan, swe_state *st, double *mass, double *e
  * operator's).  Fused plans only; blocks until the values are on the host. */
 int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7]);
 
+/* swm_output_diag_t%print_integral_tends
+ * (models/shallow_water/output_and_diag/swm_output_diag_mod.f90 L236-342) of
+ * the plan's operator: the tendency apply(st) goes to a plan-owned buffer
+ * (allocated on the first call), then out[2] = TE_tend, Enstrophy_tend with
+ * the same operators as swe_integrals_diag; hq is the first output of
+ * interp2d_uv2qvec (i2c along y of h at the u points), as there.  Fused
+ * plans only; blocks until the values are on the host. */
+int swe_integral_tends(swe_plan *plan, swe_state *st, double grav, double out[2]);
+
 /* Serving: continuous batching of independent solves of mixed size and
  * precision under a device-memory budget (no counterpart in the reference,
  * which runs one solve per program).  Requests are queued with a copy of

@@ -1191,6 +1191,81 @@ void swo_integrals_diag(swo *s, const double *u, const double *v, const double *
     out[6] = ens;
 }
 
+/* print_integral_tends (swm_output_diag_mod.f90 L236-342) of the linear
+ * operator's tendency, Ch with covariant winds, no orography: out[0] =
+ * TE_tend = sum_u w (I_x(h_t) ut u / 2 + I_x(h) ut u_t) + (the same on v) +
+ * sum_xy w g h h_t; out[1] = Enstrophy_tend = sum_o w (zeta + f) zeta_t / hq
+ * + sum_o w (-(zeta + f)^2 / 2) hq_t / hq / hq, with hq = I_y(I_x(h)) (the
+ * first output of interp2d_uv2qvec), hq_t likewise of h_t, zeta = curl(u,
+ * v), zeta_t = curl(u_t, v_t), (u_t, v_t, h_t) = apply(u, v, h).  out[2],
+ * out[3]: the sums of the terms' absolute values (the scale against which
+ * rounding in the two results is measured, both sums cancel heavily). */
+void swo_integral_tends(swo *s, const double *u, const double *v, const double *h, double grav, double out[4])
+{
+    const int N = s->N;
+    const int nu = swo_npts(s, SWO_MESH_U), nv = swo_npts(s, SWO_MESH_V), nh = swo_npts(s, SWO_MESH_XY);
+    const int no = 6 * N * N;
+    double *ut = malloc(nu * sizeof(double)), *vt = malloc(nv * sizeof(double)), *ht = malloc(nh * sizeof(double));
+    double *uc = malloc(nu * sizeof(double)), *vc = malloc(nv * sizeof(double));
+    double *hu = malloc(nu * sizeof(double)), *hv = malloc(nv * sizeof(double));
+    double *hut = malloc(nu * sizeof(double)), *hvt = malloc(nv * sizeof(double));
+    double *z = malloc(no * sizeof(double)), *zt = malloc(no * sizeof(double));
+    double *hq = malloc(no * sizeof(double)), *hqt = malloc(no * sizeof(double));
+    swo_apply(s, u, v, h, ut, vt, ht);
+    swo_co2contra(s, u, v, uc, vc);
+    /* h and h_t at the u and v points (interp2d_p2uv_Ch), then at the centres
+     * (u_q of interp2d_uv2qvec_Ch: i2c along y of the u-point values) */
+    const double *src[2] = {h, ht};
+    double *du[2] = {hu, hut}, *dv[2] = {hv, hvt}, *dq[2] = {hq, hqt};
+    for (int k = 0; k < 2; k++) {
+        fld_load(&s->h, src[k]);
+        for (int p = 0; p < 6; p++) {
+            for (int j = 1; j <= N + 1; j++) i_i2c(s->order, AT(&s->h, p, 0, j), 1, N, AT(&s->Ju, p, 0, j), 1);
+            for (int i = 1; i <= N + 1; i++)
+                i_i2c(s->order, AT(&s->h, p, i, 0), s->h.sx, N, AT(&s->Jv, p, i, 0), s->Jv.sx);
+            for (int i = 1; i <= N; i++)
+                i_i2c(s->order, AT(&s->Ju, p, i, 0), s->Ju.sx, N, AT(&s->w1, p, i, 0), s->w1.sx);
+        }
+        fld_store(&s->Ju, du[k]);
+        fld_store(&s->Jv, dv[k]);
+        for (int p = 0; p < 6; p++)
+            for (int j = 1; j <= N; j++)
+                for (int i = 1; i <= N; i++) dq[k][((size_t)p * N + (j - 1)) * N + (i - 1)] = *AT(&s->w1, p, i, j);
+    }
+    swo_curl(s, u, v, z);
+    swo_curl(s, ut, vt, zt);
+    const double *wu = s->G[SWO_MESH_U].w, *wv = s->G[SWO_MESH_V].w, *wh = s->G[SWO_MESH_XY].w;
+    double ke1 = 0, ke2 = 0, pe = 0, e1 = 0, e2 = 0, ste = 0, sen = 0;
+    if (g_mutation != 6) {
+        for (int k = 0; k < nu; k++) ke1 += 0.5 * (hut[k] * uc[k]) * u[k] * wu[k];
+        for (int k = 0; k < nv; k++) ke1 += 0.5 * (hvt[k] * vc[k]) * v[k] * wv[k];
+    }
+    for (int k = 0; k < nu; k++) ke2 += (hu[k] * uc[k]) * ut[k] * wu[k];
+    for (int k = 0; k < nv; k++) ke2 += (hv[k] * vc[k]) * vt[k] * wv[k];
+    if (g_mutation != 7)
+        for (int k = 0; k < nh; k++) pe += grav * h[k] * ht[k] * wh[k];
+    for (int k = 0; k < nu; k++) ste += (0.5 * fabs(hut[k] * uc[k] * u[k]) + fabs(hu[k] * uc[k] * ut[k])) * wu[k];
+    for (int k = 0; k < nv; k++) ste += (0.5 * fabs(hvt[k] * vc[k] * v[k]) + fabs(hv[k] * vc[k] * vt[k])) * wv[k];
+    for (int k = 0; k < nh; k++) ste += fabs(grav * h[k] * ht[k]) * wh[k];
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                const size_t id = ((size_t)p * N + (j - 1)) * N + (i - 1);
+                double J, w, f;
+                centre_geom(s, p, i, j, &J, &w, &f);
+                const double c = z[id] + (g_mutation == 9 ? 0.0 : f);
+                e1 += (c * zt[id]) / hq[id] * w;
+                if (g_mutation != 8) e2 += (((-0.5 * c * c) * hqt[id]) / hq[id]) / hq[id] * w;
+                sen += (fabs(c * zt[id] / hq[id]) + fabs(0.5 * c * c * hqt[id] / hq[id] / hq[id])) * w;
+            }
+    out[0] = (ke2 + ke1) + pe;
+    out[1] = e1 + e2;
+    out[2] = ste;
+    out[3] = sen;
+    free(ut); free(vt); free(ht); free(uc); free(vc); free(hu); free(hv); free(hut); free(hvt);
+    free(z); free(zt); free(hq); free(hqt);
+}
+
 /* test_fields_mod.f90 generate_gaussian_hill_scalar_field L491-514 (times amp) */
 void swo_gaussian_hill(const swo *s, double lam0, double phi0, double d0,
                        double amp, double *h)

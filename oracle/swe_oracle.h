@@ -69,6 +69,9 @@ double swo_mass(const swo *s, const double *h);
 void swo_curl(swo *s, const double *u, const double *v, double *curl);
 /* print_integrals_diag: hmin, hmax, mass, TE, KE, PE, enstrophy */
 void swo_integrals_diag(swo *s, const double *u, const double *v, const double *h, double grav, double out[7]);
+/* print_integral_tends of the operator's tendency: out = TE_tend,
+ * Enstrophy_tend, then the sums of their terms' absolute values */
+void swo_integral_tends(swo *s, const double *u, const double *v, const double *h, double grav, double out[4]);
 void swo_gaussian_hill(const swo *s, double lam0, double phi0, double d0,
                        double amp, double *h);
 void swo_ts2_linear(const swo *s, double u0, double h_mean, double *u,

@@ -53,6 +53,7 @@ def _load():
     lib.swo_mass.restype = ctypes.c_double
     lib.swo_curl.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     lib.swo_integrals_diag.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
+    lib.swo_integral_tends.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
     lib.swo_gaussian_hill.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]
     lib.swo_ts2_linear.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
     lib.swo_covariant_from_cart.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int, _dp]
@@ -212,6 +213,15 @@ class Oracle:
         out = np.zeros(7)
         self.lib.swo_integrals_diag(self._h, _p(u), _p(v), _p(h), grav, _p(out))
         return dict(zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), out.tolist()))
+
+    def integral_tends(self, u, v, h, grav=9.80616, scales=False):
+        """print_integral_tends of the operator's tendency: (TE_tend,
+        Enstrophy_tend), and with scales=True also the sums of their terms'
+        absolute values (the scale of their rounding)."""
+        u, v, h = self._c(u), self._c(v), self._c(h)
+        out = np.zeros(4)
+        self.lib.swo_integral_tends(self._h, _p(u), _p(v), _p(h), grav, _p(out))
+        return tuple(float(x) for x in (out if scales else out[:2]))
 
     def gaussian_hill(self, lam0, phi0, d0, amp):
         h = self.zeros(MESH_XY)

@@ -21,7 +21,7 @@
 // multi-instance launches of ensembles, the tile-wise integrals).
 //
 // usage: swe_emu <case> N order prec [steps] [seed] [flags]
-//   case  : apply | rk4 | integrals | diag | multi
+//   case  : apply | rk4 | integrals | diag | tends | multi
 //   prec  : 64 | 32
 //   flags : bit 0 every tile on the general path, bit 1 plain block order,
 //           bit 2 compensated summation (rk4)
@@ -489,6 +489,21 @@ struct Plan {
         emu::launch_flat(1, 64, [&] { k_diag_final(pp, grid, op); });
         for (int k = 0; k < 7; k++) out[k] = o[k];
     }
+
+    // Engine::integral_tends: apply into a tendency state, then the tile sums
+    template <int O>
+    void tends(T *st, T *tend, double grav, double out[2])
+    {
+        apply<O>(tend, st);
+        const int grid = std::min(ft.ntiles, INT_MAX_BLOCKS);
+        std::vector<double> prt(TENDS_K * (size_t)grid, NAN), o(2, NAN);
+        double *pp = prt.data(), *op = o.data();
+        emu::launch(grid, F_THREADS,
+                    [&] { k_tends<T, O, TY>(d, cf, g, st, tend, ft.ntx, ft.nty, ft.fwx, ft.fwy, grav, pp); });
+        emu::launch_flat(1, 64, [&] { k_tends_final(pp, grid, op); });
+        out[0] = o[0];
+        out[1] = o[1];
+    }
 };
 
 double rel(const std::vector<double> &a, const std::vector<double> &b)
@@ -587,6 +602,22 @@ int run_case(const std::string &cs, int N, int steps, unsigned seed, int flags)
             const double e = std::fabs(got[k] - ref[k]) / std::max(std::fabs(ref[k]), 1e-300);
             std::printf("diag %s %.17g oracle %.17g rel err %.3e (bar %.1e)\n", nm[k], got[k], ref[k], e, dbar);
             bad |= !(e <= dbar);
+        }
+    } else if (cs == "tends") {
+        T *st = pl.state(), *tend = pl.state();
+        Fields s1 = s0;
+        for (double &x : s1.h) x = 500.0 + 10.0 * x;
+        pl.upload(st, s1.u.data(), s1.v.data(), s1.h.data());
+        double got[2], ref[4];
+        pl.template tends<O>(st, tend, 9.80616, got);
+        swo_integral_tends(o, s1.u.data(), s1.v.data(), s1.h.data(), 9.80616, ref);
+        // measured against the sum of the terms' absolute values: both cancel
+        const double tbar = sizeof(T) == 8 ? 1e-12 : 2e-6;
+        const char *nm[2] = {"TE_tend", "Enstrophy_tend"};
+        for (int k = 0; k < 2; k++) {
+            const double e = std::fabs(got[k] - ref[k]) / ref[2 + k];
+            std::printf("tends %s %.17g oracle %.17g err/scale %.3e (bar %.1e)\n", nm[k], got[k], ref[k], e, tbar);
+            bad |= !(e <= tbar);
         }
     } else {
         std::fprintf(stderr, "unknown case %s\n", cs.c_str());
@@ -693,7 +724,7 @@ int dispatch(const std::string &cs, int N, int order, int steps, unsigned seed, 
 int main(int argc, char **argv)
 {
     if (argc < 5) {
-        std::fprintf(stderr, "usage: %s apply|rk4|integrals|diag|multi N order 64|32 [steps] [seed] [flags]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s apply|rk4|integrals|diag|tends|multi N order 64|32 [steps] [seed] [flags]\n", argv[0]);
         return 2;
     }
     const std::string cs = argv[1];

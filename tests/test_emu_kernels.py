@@ -7,7 +7,7 @@ blocks in a random order, every device buffer an exact-size heap allocation
 The cases cover single-tile panels, uneven last tiles, interior tiles, the
 general-tiles and plain-order flags, both SBP orders and precisions, the RK4
 stage modes (compensated summation included), the multi-instance (ensemble) kernels, the tile-wise
-integrals and the integral diagnostics.  No GPU is used."""
+integrals, the integral diagnostics and tendencies.  No GPU is used."""
 import os
 import shutil
 import subprocess
@@ -38,6 +38,9 @@ CASES = [
     ("diag", 41, 21, 32, 1, 16, 0),
     ("diag", 57, 42, 64, 1, 17, 0),     # with interior tiles
     ("diag", 32, 21, 64, 1, 18, 0),     # a one-point last tile along x: the centres before it read the SAT halo
+    ("tends", 20, 42, 64, 1, 19, 0),    # integral tendencies (two passes over a tile's fields)
+    ("tends", 32, 21, 64, 1, 20, 0),
+    ("tends", 57, 42, 32, 1, 21, 0),
 ]
 
 

@@ -70,6 +70,12 @@ def test_gauss_config_matches_oracle(oracle_mod, tmp_path, order, precision, tol
     dtol = 1e-10 if precision.startswith("fp64") else 1e-4
     for k, g in zip(("hmin", "hmax", "mass", "TE", "KE", "PE"), vals):
         assert g == pytest.approx(d[k], rel=dtol, abs=dtol * abs(d["hmax"])), (k, g, d[k])
+    # print_integral_tends's line after each diagnostics line (TE_tend only:
+    # the enstrophy tendency divides by the perturbation h as well)
+    tl = re.findall(r"dynamics: TE_tend = (.*), Enstrophy_tend = (.*)", stdout)
+    assert len(tl) == 2
+    te, en, ste, sen = o.integral_tends(*ref, EARTH_GRAV, scales=True)
+    assert abs(float(tl[-1][0]) - te) <= dtol * ste + 1e-11 * abs(te), (tl[-1], te, ste)
 
 
 @pytest.mark.gpu
@@ -94,6 +100,14 @@ def test_ts2_config_matches_oracle(oracle_mod, tmp_path, order):
     d = o.integrals_diag(*ref, EARTH_GRAV)
     for k, g in zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), (float(x) for x in lines[-1])):
         assert g == pytest.approx(d[k], rel=1e-10), (k, g, d[k])
+    # print_integral_tends's line ('(2(A,E20.12))': 12 digits), measured
+    # against the sum of its terms' absolute values (a steady state: both
+    # tendencies nearly cancel)
+    tl = re.findall(r"dynamics: TE_tend = (.*), Enstrophy_tend = (.*)", stdout)
+    assert len(tl) == 2
+    te, en, ste, sen = o.integral_tends(*ref, EARTH_GRAV, scales=True)
+    assert abs(float(tl[-1][0]) - te) <= 1e-10 * ste + 1e-11 * abs(te), (tl[-1], te, ste)
+    assert abs(float(tl[-1][1]) - en) <= 1e-10 * sen + 1e-11 * abs(en), (tl[-1], en, sen)
     errs = re.findall(r"Errors, t=\s*(\S+)\s+hours, l2_h =\s*(\S+)\s*, linf_h =\s*(\S+)", stdout)
     assert [float(e[0]) for e in errs] == [0.0, 1.0, 2.0]
     assert float(errs[0][1]) == 0.0 and 0 < float(errs[2][1]) < 1e-3

@@ -393,3 +393,27 @@ def test_integrals_diag_matches_oracle(oracle_mod, order, N, precision, omega):
     tol = 1e-12 if precision == cs.FP64 else 2e-6
     for k, r in ref.items():
         assert abs(got[k] - r) <= tol * abs(r), (k, got[k], r)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,N,precision,omega", DIAG_CASES)
+def test_integral_tends_match_oracle(oracle_mod, order, N, precision, omega):
+    """swe_integral_tends (print_integral_tends's TE_tend and Enstrophy_tend
+    of the operator's tendency) against the oracle on a random state with
+    winds of 10 m/s over a deep layer.  Both results are sums that cancel, so
+    the error is measured against the sum of their terms' absolute values:
+    FP64 1e-12, FP32 2e-6 of it."""
+    _say(f"integral_tends order={order} N={N} precision={precision} omega={omega}")
+    o = oracle_mod.Oracle(N, order, radius=6371229.0, omega=omega)
+    u, v, h = _random_state(o, 60 + N)
+    u, v, h = 10.0 * u, 10.0 * v, 500.0 + 10.0 * h
+    op = _op(N, order, precision, radius=6371229.0, omega=omega)
+    st = op.create_stvec().upload(u, v, h)
+    got = op.integral_tends(st, 9.80616)
+    te, en, ste, sen = o.integral_tends(u, v, h, 9.80616, scales=True)
+    tol = 1e-12 if precision == cs.FP64 else 2e-6
+    assert abs(got["TE_tend"] - te) <= tol * ste, (got, te, ste)
+    assert abs(got["Enstrophy_tend"] - en) <= tol * sen, (got, en, sen)
+    # the state is left as it was
+    uu, vv, hh = st.download()
+    assert np.array_equal(hh, h) if precision == cs.FP64 else np.allclose(hh, h, rtol=1e-6)

@@ -152,3 +152,81 @@ def test_diag_extrema_and_mass_follow_h(oracle_mod):
     assert d["hmin"] == h.min() and d["hmax"] == h.max()
     assert d["mass"] == o.mass(h)
     assert d["TE"] == d["KE"] + d["PE"]
+
+
+def _fd5(f, eps):
+    """Five-point derivative at 0 (exact for quartics)."""
+    return (8 * (f(eps) - f(-eps)) - (f(2 * eps) - f(-2 * eps))) / (12 * eps)
+
+
+def _tends_state(o, seed, h_var):
+    rng = np.random.default_rng(seed)
+    u, v = 10 * rng.standard_normal(o.shape(MESH_U)), 10 * rng.standard_normal(o.shape(MESH_V))
+    h = 500.0 + h_var * rng.standard_normal(o.shape(MESH_XY))
+    return u, v, h
+
+
+@pytest.mark.parametrize("order", [21, 42])
+def test_tends_are_derivatives_of_the_diagnostics(oracle_mod, order):
+    """print_integral_tends against the time derivative of
+    print_integrals_diag along the operator's own tendency F = apply(x),
+    taken by finite differences of the oracle's diagnostics at x + eps F.
+
+    Enstrophy: d/dt sum w (z + f)^2 / (2 hq) is exactly the two terms
+    print_integral_tends sums, and hq from the u path alone equals the
+    diagnostics' average of the u and v paths (the i2c interpolations along x
+    and y commute), for any h.  TE: with a constant layer h = c (I(h) = c) the
+    KE part is c <co2contra(u), u_t> + I(h_t) terms, the derivative of
+    c <co2contra(u), u> / 2 because the Ch co2contra is symmetric in the u/v
+    quadrature (checked here too); for a varying layer only the PE part
+    stays exact, checked against the dropped-PE variant's error.  Planted
+    mistakes (a dropped KE term, PE term, or hq_t term, or f left out of the
+    absolute vorticity) each fail one of these."""
+    o = oracle_mod.Oracle(16, order, radius=A, omega=7.292e-5, grav=9.80616, h_mean=1000.0)
+    g = 9.80616
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal(o.shape(MESH_U)), rng.standard_normal(o.shape(MESH_V))
+    b = rng.standard_normal(o.shape(MESH_U)), rng.standard_normal(o.shape(MESH_V))
+    ca, cb = o.co2contra(*a), o.co2contra(*b)
+    wu, wv = o.weights(MESH_U).reshape(o.shape(MESH_U)), o.weights(MESH_V).reshape(o.shape(MESH_V))
+    s1 = np.sum(wu * ca[0] * b[0]) + np.sum(wv * ca[1] * b[1])
+    s2 = np.sum(wu * cb[0] * a[0]) + np.sum(wv * cb[1] * a[1])
+    assert abs(s1 - s2) < 1e-13 * abs(s1)
+
+    def check(u, v, h, mutation=0):
+        ut, vt, ht = o.apply(u, v, h)
+        eps = 1e-3 * 500 / np.abs(ht).max()
+        d = {k: _fd5(lambda e: o.integrals_diag(u + e * ut, v + e * vt, h + e * ht, g)[k], eps)
+             for k in ("TE", "Enstrophy")}
+        oracle_mod.set_mutation(mutation)
+        try:
+            te, en = o.integral_tends(u, v, h, g)
+        finally:
+            oracle_mod.set_mutation(0)
+        wh = o.weights(MESH_XY).reshape(h.shape)
+        pe_scale = g * np.sum(wh * np.abs(h * ht))
+        return abs(d["TE"] - te) / pe_scale, abs(d["Enstrophy"] - en) / abs(en)
+
+    uc, vc, hc = _tends_state(o, 12, 0.0)
+    uvar, vvar, hvar = _tends_state(o, 13, 100.0)
+    te_c, en_c = check(uc, vc, hc)
+    te_v, en_v = check(uvar, vvar, hvar)
+    assert te_c < 1e-9 and en_c < 1e-7 and en_v < 1e-7, (te_c, en_c, en_v)
+    assert te_v < 2e-4, te_v  # the KE terms' asymmetry under a varying I(h)
+    assert check(uc, vc, hc, 6)[0] > 1e3 * max(te_c, 1e-12)   # KE term of h_t dropped
+    assert check(uvar, vvar, hvar, 7)[0] > 20 * te_v           # PE term dropped
+    assert check(uc, vc, hc, 8)[1] > 0.1                        # hq_t term dropped
+    assert check(uvar, vvar, hvar, 9)[1] > 0.1                  # f left out
+
+
+def test_tends_at_rest_and_scales(oracle_mod):
+    """At rest with a constant layer the tendency vanishes (up to the
+    rounding of a gradient of a constant) and so do both results; the
+    returned scales bound the results."""
+    o = oracle_mod.Oracle(12, 42, radius=A, omega=7.292e-5)
+    z = (o.zeros(MESH_U), o.zeros(MESH_V))
+    te, en, ste, sen = o.integral_tends(*z, np.full(o.shape(MESH_XY), 300.0), scales=True)
+    assert abs(te) < 1e-30 and abs(en) < 1e-50, (te, en)
+    u, v, h = _tends_state(o, 14, 30.0)
+    te, en, ste, sen = o.integral_tends(u, v, h, scales=True)
+    assert abs(te) <= ste and abs(en) <= sen and ste > 0 and sen > 0
