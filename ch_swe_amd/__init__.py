@@ -38,7 +38,7 @@ EXPORTS = (
     "swe_last_error", "swe_version", "swe_plan_create", "swe_plan_destroy", "swe_plan_npts",
     "swe_plan_stream", "swe_synchronize", "swe_state_create", "swe_state_destroy",
     "swe_state_upload", "swe_state_download", "swe_apply", "swe_assign", "swe_update",
-    "swe_rk4_steps", "swe_integrals", "swe_integrals_noalloc", "swe_integrals_diag", "swe_integral_tends", "swe_halo_map", "swe_ensemble_create",
+    "swe_rk4_steps", "swe_integrals", "swe_integrals_noalloc", "swe_integrals_diag", "swe_integral_tends", "swe_latlon_h", "swe_halo_map", "swe_ensemble_create",
     "swe_ensemble_destroy", "swe_ensemble_rk4", "swe_ensemble_stream", "swe_ensemble_synchronize",
     "swe_plan_timing", "swe_plan_kernel_times", "swe_server_create", "swe_server_destroy",
     "swe_server_submit", "swe_server_tick", "swe_server_wait", "swe_server_take", "swe_server_stats",
@@ -107,6 +107,7 @@ def lib():
     L.swe_integrals_noalloc.argtypes = [_vp, _vp, _dp, _dp]
     L.swe_integrals_diag.argtypes = [_vp, _vp, ctypes.c_double, _dp]
     L.swe_integral_tends.argtypes = [_vp, _vp, ctypes.c_double, _dp]
+    L.swe_latlon_h.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     L.swe_halo_map.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip] + [_ip] * 9
     L.swe_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
     L.swe_ensemble_destroy.argtypes = [_vp]
@@ -245,6 +246,17 @@ class StvecSwm:
         _check(lib().swe_update(self._plan.h, self.h, s1, v1.h, s2, v2.h if v2 is not None else None))
 
 
+def write_latlon_record(path, field, rec):
+    """Write one lat-lon field as record rec (1-based) of a direct-access
+    float32 file, as latlon_write does (latlon_outputer_mod.f90 L52-60: record
+    length nlat * nlon values, longitude fastest)."""
+    a = np.ascontiguousarray(field, dtype=np.float32)
+    mode = "r+b" if os.path.exists(path) else "wb"
+    with open(path, mode) as f:
+        f.seek((int(rec) - 1) * a.nbytes)
+        f.write(a.tobytes())
+
+
 class OperatorSwmLin:
     """operator_swm_lin_t: the linear SWE tendency, apply only (explicit
     schemes never call solve)."""
@@ -271,6 +283,17 @@ class OperatorSwmLin:
         out = np.zeros(7)
         _check(lib().swe_integrals_diag(self.plan.h, state.h, float(grav), _p(out)))
         return dict(zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), out.tolist()))
+
+    def latlon_h(self, state, nlat=None, nlon=None):
+        """h on the lat-lon grid as write_fields writes h.dat
+        (swm_output_diag_mod.f90 L52-73, latlon_outputer_mod.f90 L33-63):
+        float32 (nlat, nlon), default 2N+1 latitudes x 4N longitudes."""
+        N = self.cfg.N
+        nlat = 2 * N + 1 if nlat is None else int(nlat)
+        nlon = 4 * N if nlon is None else int(nlon)
+        out = np.empty((nlat, nlon), np.float32)
+        _check(lib().swe_latlon_h(self.plan.h, state.h, nlat, nlon, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
 
     def integral_tends(self, state, grav=9.80616):
         """print_integral_tends (swm_output_diag_mod.f90 L236-342) of this

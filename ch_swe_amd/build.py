@@ -9,7 +9,7 @@ LIB = os.path.join(HERE, "libchswe.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["swe_lib.hip", "swe_topology.cpp", "swe_config.cpp"]
+SOURCES = ["swe_lib.hip", "swe_topology.cpp", "swe_config.cpp", "swe_latlon.cpp"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep the compiler from forming packed FP32 instructions

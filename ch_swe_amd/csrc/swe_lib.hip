@@ -37,6 +37,7 @@
 #include "swe_sbp.hpp"
 #include "swe_error.hpp"
 #include "swe_topology.hpp"
+#include "swe_latlon.hpp"
 
 namespace {
 
@@ -304,6 +305,7 @@ __global__ void k_reduce_final(const double *part, int nb, double *out)
 
 #include "swe_fused.inc"
 #include "swe_fused_host.hpp"
+#include "swe_latlon.inc"
 
 // ------------------------------------------------------------------------
 // engine
@@ -390,6 +392,8 @@ struct EngineBase {
     virtual int integrals_diag(void *st, double grav, double out[7]) = 0;
     // print_integral_tends's values of the operator's tendency: TE_tend, Enstrophy_tend
     virtual int integral_tends(void *st, double grav, double out[2]) = 0;
+    // h on the nlat x nlon lat-lon grid as the output writes h.dat (host out)
+    virtual int latlon_h(void *st, int nlat, int nlon, float *out) = 0;
     virtual int rk4(void *v0, void *acc, void *ya, void *yb, double dt, int nsteps) = 0;
     virtual bool fused() const = 0;
     virtual size_t elem() const = 0;
@@ -457,6 +461,17 @@ struct Engine : EngineBase {
     {
         if (graph) (void)hipGraphExecDestroy(graph);
         for (void *p : allocs) (void)hipFree(p);
+    }
+
+    // hipFree of a buffer from alloc
+    void release(void *p)
+    {
+        for (size_t k = 0; k < allocs.size(); k++)
+            if (allocs[k] == p) {
+                allocs.erase(allocs.begin() + k);
+                (void)hipFree(p);
+                return;
+            }
     }
 
     int alloc(void **p, size_t nbytes)
@@ -791,6 +806,69 @@ struct Engine : EngineBase {
         else launch_diag<42>((const T *)st, grav);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(out, dout, 7 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        return SWE_OK;
+    }
+
+    // lat-lon output: the tables of the last grid asked for, the extended
+    // arrays and the output (built on the first call for a grid)
+    struct {
+        int nlat = 0, nlon = 0, nrows = 0, np = 0, ew = 0;
+        size_t nbytes = 0;
+        int *rp = nullptr, *col = nullptr, *base = nullptr;
+        double *w = nullptr, *fx = nullptr, *fy = nullptr, *ext = nullptr;
+        float *out = nullptr;
+    } ll;
+    int latlon_tables(int nlat, int nlon)
+    {
+        if (ll.nlat == nlat && ll.nlon == nlon) return SWE_OK;
+        swe::LatlonTables t;
+        std::string err;
+        if (swe::build_latlon_tables(d.N, nlat, nlon, t, err)) return fail(SWE_EINVAL, "swe_latlon_h: " + err);
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        void *old[] = {ll.rp, ll.col, ll.base, ll.w, ll.fx, ll.fy, ll.ext, ll.out};
+        for (void *q : old)
+            if (q) release(q);
+        bytes -= ll.nbytes;
+        ll = {};
+        const int n = d.N + 1;
+        std::vector<int> col(t.src.size());
+        for (size_t k = 0; k < col.size(); k++) {
+            const int s = t.src[k], p = s / (n * n), r = s - p * n * n, j = r / n + 1, i = r - (j - 1) * n + 1;
+            col[k] = (int)off(d, p, i, j);
+        }
+        const size_t nr = t.rp.size() - 1, np = t.base.size(), nnz = col.size();
+        const size_t before = bytes;
+        int rc;
+        if ((rc = alloc((void **)&ll.rp, (nr + 1) * sizeof(int))) || (rc = alloc((void **)&ll.col, nnz * sizeof(int))) ||
+            (rc = alloc((void **)&ll.w, nnz * sizeof(double))) || (rc = alloc((void **)&ll.base, np * sizeof(int))) ||
+            (rc = alloc((void **)&ll.fx, np * sizeof(double))) || (rc = alloc((void **)&ll.fy, np * sizeof(double))) ||
+            (rc = alloc((void **)&ll.ext, nr * sizeof(double))) || (rc = alloc((void **)&ll.out, np * sizeof(float))))
+            return rc;
+        HIP_TRY(hipMemcpyAsync(ll.rp, t.rp.data(), (nr + 1) * sizeof(int), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(ll.col, col.data(), nnz * sizeof(int), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(ll.w, t.w.data(), nnz * sizeof(double), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(ll.base, t.base.data(), np * sizeof(int), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(ll.fx, t.fx.data(), np * sizeof(double), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(ll.fy, t.fy.data(), np * sizeof(double), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        ll.nlat = nlat;
+        ll.nlon = nlon;
+        ll.nrows = (int)nr;
+        ll.np = (int)np;
+        ll.ew = t.ew;
+        ll.nbytes = bytes - before;
+        return SWE_OK;
+    }
+    int latlon_h(void *st, int nlat, int nlon, float *out) override
+    {
+        int rc = latlon_tables(nlat, nlon);
+        if (rc) return rc;
+        const T *h = (const T *)st + 2 * d.F;
+        k_ll_ext<T><<<(ll.nrows + 255) / 256, 256, 0, pl->stream>>>(h, ll.rp, ll.col, ll.w, ll.nrows, ll.ext);
+        k_ll_interp<<<(ll.np + 255) / 256, 256, 0, pl->stream>>>(ll.ext, ll.base, ll.fx, ll.fy, ll.np, ll.ew, ll.out);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, ll.out, (size_t)ll.np * sizeof(float), hipMemcpyDeviceToHost, pl->stream));
         HIP_TRY(hipStreamSynchronize(pl->stream));
         return SWE_OK;
     }
@@ -1179,6 +1257,14 @@ int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7]
     int rc = check_state(plan, st, "state");
     if (rc) return rc;
     return plan->eng->integrals_diag(st->buf, grav, out);
+}
+
+int swe_latlon_h(swe_plan *plan, swe_state *st, int nlat, int nlon, float *out)
+{
+    if (!plan || !out) return fail(SWE_EINVAL, "swe_latlon_h: NULL argument");
+    int rc = check_state(plan, st, "state");
+    if (rc) return rc;
+    return plan->eng->latlon_h(st->buf, nlat, nlon, out);
 }
 
 int swe_integral_tends(swe_plan *plan, swe_state *st, double grav, double out[2])

@@ -1,7 +1,7 @@
 ! iso_c_binding interface to libchswe.so (include/ch_swe.h).
 module swe_hip_capi_mod
 
-    use iso_c_binding, only : c_int, c_double, c_ptr, c_char, c_null_char, c_associated, c_f_pointer
+    use iso_c_binding, only : c_int, c_double, c_float, c_ptr, c_char, c_null_char, c_associated, c_f_pointer
 
     implicit none
 
@@ -28,7 +28,8 @@ module swe_hip_capi_mod
 
     public :: swe_plan_create, swe_plan_destroy, swe_state_create, swe_state_destroy, &
               swe_state_upload, swe_state_download, swe_apply, swe_assign, swe_update,  &
-              swe_rk4_steps, swe_integrals, swe_integrals_diag, swe_integral_tends, swe_synchronize, swe_error_message
+              swe_rk4_steps, swe_integrals, swe_integrals_diag, swe_integral_tends, swe_latlon_h, &
+              swe_synchronize, swe_error_message
 
     interface
         integer(c_int) function swe_plan_create(cfg, plan) bind(C, name="swe_plan_create")
@@ -93,6 +94,12 @@ module swe_hip_capi_mod
             type(c_ptr), value :: plan, st
             real(c_double), value :: grav
             real(c_double), intent(out) :: out(7)
+        end function
+        integer(c_int) function swe_latlon_h(plan, st, nlat, nlon, out) bind(C, name="swe_latlon_h")
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: plan, st
+            integer(c_int), value :: nlat, nlon
+            real(c_float), intent(out) :: out(*)
         end function
         integer(c_int) function swe_integral_tends(plan, st, grav, out) bind(C, name="swe_integral_tends")
             import :: c_int, c_ptr, c_double

@@ -107,6 +107,7 @@ module swe_hip_mod
         procedure :: integrals => operator_integrals
         procedure :: integrals_diag => operator_integrals_diag
         procedure :: integral_tends => operator_integral_tends
+        procedure :: latlon_h => operator_latlon_h
     end type operator_swm_lin_hip_t
 
     type, extends(timescheme_t) :: rk4_hip_t
@@ -291,6 +292,18 @@ contains
         real(kind=8),                  intent(out)   :: tends(2)
         call check(swe_integral_tends(domain%plan, hip_state(v), grav, tends), "integral_tends")
     end subroutine operator_integral_tends
+
+    ! swm_output_diag_mod.f90 write_fields L52-73 (h_surf = 0) through
+    ! latlon_outputer_mod.f90 latlon_write L33-63: h on the lat-lon grid,
+    ! hll(nlon, nlat) as one record of h.dat
+    subroutine operator_latlon_h(this, v, domain, hll)
+        class(operator_swm_lin_hip_t), intent(inout) :: this
+        class(stvec_t),                intent(in)    :: v
+        type(domain_hip_t),            intent(in)    :: domain
+        real(kind=4),                  intent(out)   :: hll(:, :)
+        call check(swe_latlon_h(domain%plan, hip_state(v), int(size(hll, 2), c_int), int(size(hll, 1), c_int), hll), &
+                   "latlon_h")
+    end subroutine operator_latlon_h
 
     ! rk4_mod.f90 step_rk4 L20-42 (fused stages on the device)
     subroutine rk4_step(this, v0, operator, domain, dt)

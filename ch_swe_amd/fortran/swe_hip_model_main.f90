@@ -34,8 +34,10 @@
 ! TE, KE, PE, enstrophy, computed on the device (swe_integrals_diag) with the
 ! Ch operators of the paper's output_diag section, followed by the line of
 ! print_integral_tends (L236-342, "dynamics: TE_tend = ..., Enstrophy_tend =
-! ...", swe_integral_tends) (the lat-lon h.dat fields are not produced by this
-! driver).
+! ...", swe_integral_tends).  h.dat in the working directory gets the lat-lon
+! h (2N+1 latitudes, 4N longitudes, float32; swe_latlon_h) at t = 0 and at
+! every write step, as write_fields writes it (u.dat, v.dat are not
+! produced).
 ! The final state is written to out.bin (default swe_hip_state.bin) in the
 ! packed (u, v, h) layout of include/ch_swe.h.
 program swe_hip_model
@@ -67,6 +69,7 @@ program swe_hip_model
     real(kind=8) :: dt, tau_write, tau_diagnostics, simulation_time, radius, omega, h_mean, axis(3), rm(9)
     real(kind=8) :: lam0, phi0, d0, u0, tc_h_mean, an(3), c(3), r(3), al, be, X, Y, s, vel(3), a1(3), a2(3)
     real(kind=8) :: diag(7), tends(2), l2_h0, linf_h0
+    real(kind=4), allocatable :: hll(:, :)
     logical :: coriolis_const, linear_mode, gauss, have_refsol
 
     call get_command_argument(1, cfg_file)
@@ -238,6 +241,10 @@ program swe_hip_model
     call create_stvec_hip(state, domain)
     call state%upload(u, v, h)
 
+    ! write_fields (swm_output_diag_mod.f90 L52-73): h.dat record 1 holds the
+    ! initial state, record k + 1 the state after k write intervals
+    allocate(hll(4 * N, 2 * N + 1))
+    call write_h(1)
     call print_scores(0)
     it = 0
     do while (it < nsteps)
@@ -259,7 +266,10 @@ program swe_hip_model
                 call print_scores(it)
             end if
         end if
-        if (mod(it, nstep_write) == 0) print *, "rec", it / nstep_write + 1
+        if (mod(it, nstep_write) == 0) then
+            call write_h(it / nstep_write + 1)
+            print *, "rec", it / nstep_write + 1
+        end if
     end do
     call state%download(u, v, h)
 
@@ -375,5 +385,22 @@ contains
         end if
         write(*, '(A)') trim(str_buff)
     end subroutine print_scores
+
+    ! latlon_outputer_mod.f90 latlon_write L33-63: record irec of the
+    ! direct-access float32 file h.dat (record length nlat * nlon values,
+    ! longitude fastest); the file is started afresh with record 1
+    subroutine write_h(irec)
+        integer, intent(in) :: irec
+        integer :: unit, rl
+        call operator%latlon_h(state, domain, hll)
+        inquire(iolength=rl) hll
+        if (irec == 1) then
+            open(newunit=unit, file="h.dat", access="direct", form="unformatted", recl=rl, status="replace")
+        else
+            open(newunit=unit, file="h.dat", access="direct", form="unformatted", recl=rl, status="old")
+        end if
+        write(unit, rec=irec) hll
+        close(unit)
+    end subroutine write_h
 
 end program swe_hip_model

@@ -134,6 +134,19 @@ int swe_integrals_noalloc(swe_plan *plan, swe_state *st, double *mass, double *e
  * operator's).  Fused plans only; blocks until the values are on the host. */
 int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7]);
 
+/* The lat-lon output of h: swm_output_diag_t%write_fields's h.dat
+ * (models/shallow_water/output_and_diag/swm_output_diag_mod.f90 L52-73,
+ * h_surf = 0) through latlon_outputer_t%write
+ * (outputer/latlon_outputer_mod.f90 L33-63) with the cubic lat-lon regrid
+ * of the xy points (regridders/latlon_regrid_mod.f90, ECS_xy halo of width
+ * 2, equiang_cs/ecs_halo_xy_mod.f90).  out gets nlat * nlon floats,
+ * longitude fastest: latitude -90 + (j-1) 180 / (nlat-1) degrees, longitude
+ * (i-1) 360 / nlon degrees -- one record of h.dat (the reference writes
+ * nlat = 2N+1, nlon = 4N).  The plan's tables for the grid are built on the
+ * first call for it (device memory grows by about 60 bytes per lat-lon
+ * point); blocks until out is filled. */
+int swe_latlon_h(swe_plan *plan, swe_state *st, int nlat, int nlon, float *out);
+
 /* swm_output_diag_t%print_integral_tends
  * (models/shallow_water/output_and_diag/swm_output_diag_mod.f90 L236-342) of
  * the plan's operator: the tendency apply(st) goes to a plan-owned buffer

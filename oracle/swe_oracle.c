@@ -1266,6 +1266,150 @@ void swo_integral_tends(swo *s, const double *u, const double *v, const double *
     free(z); free(zt); free(hq); free(hqt);
 }
 
+/* ---- lat-lon output of h (latlon_outputer_mod.f90 latlon_write L33-63) ---- */
+
+/* init_halo_xy_interp (ecs_halo_factory_mod.f90 L230-269): the virtual
+ * point at along-edge index i, depth j beyond a panel edge, projected onto
+ * the neighbour's row j inside; cubic weights over ind-1 .. ind+2 */
+static void ecs_xy_weights(int n, double hx, int i, int j, double w[4], int *ind)
+{
+    const double zb = 0.25 * PI + j * hx, za0 = -0.25 * PI + (i - 1.0) * hx;
+    const double za = atan(tan(za0) / tan(zb));
+    int k = (int)floor((za + 0.25 * PI) / hx) + 1;
+    k = k < 2 ? 2 : k;
+    k = k > n - 2 ? n - 2 : k;
+    const double z = (za + 0.25 * PI) / hx - (k - 1.0);
+    w[0] = -z * (z - 1.0) * (z - 2.0) / 6.0;
+    w[1] = (z + 1.0) * (z - 1.0) * (z - 2.0) / 2.0;
+    w[2] = -(z + 1.0) * z * (z - 2.0) / 2.0;
+    w[3] = (z + 1.0) * z * (z - 1.0) / 6.0;
+    *ind = k;
+}
+
+/* ext_ecs_tile_halo (ecs_halo_xy_mod.f90 L88-203) on one whole panel of the
+ * exchanged field E (xy halo depth hw + 1: the neighbour's duplicate of the
+ * edge at depth 1, its row j inside at depth j + 1): the corners' three
+ * copies averaged and their diagonals set, then edges bottom, top, left,
+ * right: edge points averaged with the duplicate, the virtual rows 1..hw
+ * interpolated along the neighbour's rows. */
+static void ecs_xy_halo_panel(fld *E, int p, int n, double hx, int hw)
+{
+#define F(i, j) (*AT(E, p, (i), (j)))
+    /* corners 1 (1,1), 2 (n,1), 3 (n,n), 4 (1,n) */
+    F(1, 1) = (F(1, 1) + F(1, 0) + F(0, 1)) / 3.0;
+    for (int j = 1; j <= hw; j++) {
+        F(1 - j, 1 - j) = 0.5 * (F(-j, 1) + F(1, -j));
+        F(-j, 1) = F(1 - j, 1 - j);
+        F(1, -j) = F(1 - j, 1 - j);
+    }
+    F(n, 1) = (F(n, 1) + F(n, 0) + F(n + 1, 1)) / 3.0;
+    for (int j = 1; j <= hw; j++) {
+        F(n + j, 1 - j) = 0.5 * (F(n + j + 1, 1) + F(n, -j));
+        F(n + j + 1, 1) = F(n + j, 1 - j);
+        F(n, -j) = F(n + j, 1 - j);
+    }
+    F(n, n) = (F(n, n) + F(n, n + 1) + F(n + 1, n)) / 3.0;
+    for (int j = 1; j <= hw; j++) {
+        F(n + j, n + j) = 0.5 * (F(n + j + 1, n) + F(n, n + j + 1));
+        F(n + j + 1, n) = F(n + j, n + j);
+        F(n, n + j + 1) = F(n + j, n + j);
+    }
+    F(1, n) = (F(1, n) + F(0, n) + F(1, n + 1)) / 3.0;
+    for (int j = 1; j <= hw; j++) {
+        F(1 - j, n + j) = 0.5 * (F(1, n + j + 1) + F(-j, n));
+        F(1, n + j + 1) = F(1 - j, n + j);
+        F(-j, n) = F(1 - j, n + j);
+    }
+    double *buf = malloc((size_t)(n + 2 * HALO) * sizeof(double)), *zb = buf + HALO - 1; /* zb[k], k >= 1 - HALO */
+    double w[4];
+    int ind;
+    for (int e = 0; e < 4; e++) {
+        /* edge 1 bottom, 2 top, 3 left, 4 right; along-edge index a, depth j */
+        if (g_mutation != 10)
+            for (int a = 2; a <= n - 1; a++) {
+                double *x = e == 0 ? &F(a, 1) : e == 1 ? &F(a, n) : e == 2 ? &F(1, a) : &F(n, a);
+                const double y = e == 0 ? F(a, 0) : e == 1 ? F(a, n + 1) : e == 2 ? F(0, a) : F(n + 1, a);
+                *x = 0.5 * (*x + y);
+            }
+        for (int j = 1; j <= hw; j++) {
+            const int jr = g_mutation == 11 ? j - 1 : j; /* 11: the neighbour's row one short */
+            for (int a = 1; a <= n; a++)
+                zb[a] = e == 0 ? F(a, -jr) : e == 1 ? F(a, n + jr + 1) : e == 2 ? F(-jr, a) : F(n + jr + 1, a);
+            for (int a = 2 - j; a <= n + j - 1; a++) {
+                ecs_xy_weights(n, hx, a, j, w, &ind);
+                const double v = w[0] * zb[ind - 1] + w[1] * zb[ind] + w[2] * zb[ind + 1] + w[3] * zb[ind + 2];
+                if (e == 0) F(a, 1 - j) = v;
+                else if (e == 1) F(a, n + j) = v;
+                else if (e == 2) F(1 - j, a) = v;
+                else F(n + j, a) = v;
+            }
+        }
+    }
+    free(buf);
+#undef F
+}
+
+/* h on the lat-lon grid (latlon_grid_generator_mod.f90 init /
+ * get_cartesian_coords: lam_i = (i-1) 2 pi / nlon, phi_j = -pi/2 + (j-1) pi /
+ * (nlat-1)), as the shallow-water output writes h.dat
+ * (swm_output_diag_mod.f90 write_fields L52-73 with h_surf = 0,
+ * create_latlon_outputer: cubic): the exchanged, ECS-halo'd xy field
+ * (ECS_xy halo, width 2), each point's panel the first of maximal -n.r and
+ * its (alpha, beta) by transform_cartesian_to_native_ecs
+ * (metric_2d_ecs_mod.f90 L310-336), cubic Lagrange weights at x_ind =
+ * int(zdx), zdx = (alpha + pi/4)/hx + 1 (latlon_regrid_mod.f90
+ * init_interpolation L298-326, interpolate L330-363), rounded to float as
+ * latlon_write stores it.  out[(j-1) nlon + (i-1)]. */
+void swo_latlon_h(swo *s, const double *h, int nlat, int nlon, float *out)
+{
+    const int N = s->N, n = N + 1, hw = 2;
+    const double hx = 0.5 * PI / N;
+    hmap m = {0, 0};
+    build_xy_map(&m, N, hw + 1, 0);
+    fld E;
+    fld_init(&E, n, n);
+    fld_load(&E, h);
+    apply_map(&m, &E, &E);
+    for (int p = 0; p < 6; p++) ecs_xy_halo_panel(&E, p, n, hx, hw);
+    for (int j = 1; j <= nlat; j++) {
+        const double phi = -0.5 * PI + (j - 1) * PI / (nlat - 1.0);
+        for (int i = 1; i <= nlon; i++) {
+            const double lam = (i - 1) * 2.0 * PI / nlon;
+            const double r[3] = {cos(phi) * cos(lam), cos(phi) * sin(lam), sin(phi)};
+            int pan = 0;
+            double rn = 0.0;
+            for (int q = 0; q < 6; q++) {
+                const double d = -(r[0] * T_N[q][0] + r[1] * T_N[q][1] + r[2] * T_N[q][2]);
+                if (d > rn) {
+                    pan = q;
+                    rn = d;
+                }
+            }
+            const double al = atan((T_EX[pan][0] * r[0] + T_EX[pan][1] * r[1] + T_EX[pan][2] * r[2]) / rn);
+            const double be = atan((T_EY[pan][0] * r[0] + T_EY[pan][1] * r[1] + T_EY[pan][2] * r[2]) / rn);
+            double zx = (al + 0.25 * PI) / hx + 1.0, zy = (be + 0.25 * PI) / hx + 1.0;
+            if (g_mutation == 12) { /* 12: alpha and beta exchanged */
+                const double t = zx;
+                zx = zy;
+                zy = t;
+            }
+            const int xi = (int)zx, yi = (int)zy;
+            zx -= xi;
+            zy -= yi;
+            const double wx[4] = {-zx * (zx - 1.0) * (zx - 2.0) / 6.0, (zx + 1.0) * (zx - 1.0) * (zx - 2.0) / 2.0,
+                                  -(zx + 1.0) * zx * (zx - 2.0) / 2.0, (zx + 1.0) * zx * (zx - 1.0) / 6.0};
+            const double wy[4] = {-zy * (zy - 1.0) * (zy - 2.0) / 6.0, (zy + 1.0) * (zy - 1.0) * (zy - 2.0) / 2.0,
+                                  -(zy + 1.0) * zy * (zy - 2.0) / 2.0, (zy + 1.0) * zy * (zy - 1.0) / 6.0};
+            double v = 0.0;
+            for (int b = -1; b <= 2; b++)
+                for (int a = -1; a <= 2; a++) v = v + wx[a + 1] * wy[b + 1] * *AT(&E, pan, xi + a, yi + b);
+            out[(size_t)(j - 1) * nlon + (i - 1)] = (float)v;
+        }
+    }
+    fld_free(&E);
+    free(m.e);
+}
+
 /* test_fields_mod.f90 generate_gaussian_hill_scalar_field L491-514 (times amp) */
 void swo_gaussian_hill(const swo *s, double lam0, double phi0, double d0,
                        double amp, double *h)

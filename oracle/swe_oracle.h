@@ -69,6 +69,9 @@ double swo_mass(const swo *s, const double *h);
 void swo_curl(swo *s, const double *u, const double *v, double *curl);
 /* print_integrals_diag: hmin, hmax, mass, TE, KE, PE, enstrophy */
 void swo_integrals_diag(swo *s, const double *u, const double *v, const double *h, double grav, double out[7]);
+/* h (packed xy layout) on the nlat x nlon lat-lon grid as the shallow-water
+ * output writes h.dat: float, longitude fastest */
+void swo_latlon_h(swo *s, const double *h, int nlat, int nlon, float *out);
 /* print_integral_tends of the operator's tendency: out = TE_tend,
  * Enstrophy_tend, then the sums of their terms' absolute values */
 void swo_integral_tends(swo *s, const double *u, const double *v, const double *h, double grav, double out[4]);

@@ -54,6 +54,7 @@ def _load():
     lib.swo_curl.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     lib.swo_integrals_diag.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
     lib.swo_integral_tends.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
+    lib.swo_latlon_h.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     lib.swo_gaussian_hill.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]
     lib.swo_ts2_linear.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
     lib.swo_covariant_from_cart.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int, _dp]
@@ -222,6 +223,16 @@ class Oracle:
         out = np.zeros(4)
         self.lib.swo_integral_tends(self._h, _p(u), _p(v), _p(h), grav, _p(out))
         return tuple(float(x) for x in (out if scales else out[:2]))
+
+    def latlon_h(self, h, nlat=None, nlon=None):
+        """h on the lat-lon grid as the shallow-water output writes h.dat
+        (default 2N+1 latitudes x 4N longitudes): float32 (nlat, nlon)."""
+        nlat = 2 * self.N + 1 if nlat is None else nlat
+        nlon = 4 * self.N if nlon is None else nlon
+        h = self._c(h)
+        out = np.zeros((nlat, nlon), np.float32)
+        self.lib.swo_latlon_h(self._h, _p(h), nlat, nlon, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return out
 
     def gaussian_hill(self, lam0, phi0, d0, amp):
         h = self.zeros(MESH_XY)

@@ -21,7 +21,7 @@
 // multi-instance launches of ensembles, the tile-wise integrals).
 //
 // usage: swe_emu <case> N order prec [steps] [seed] [flags]
-//   case  : apply | rk4 | integrals | diag | tends | multi
+//   case  : apply | rk4 | integrals | diag | tends | latlon | multi
 //   prec  : 64 | 32
 //   flags : bit 0 every tile on the general path, bit 1 plain block order,
 //           bit 2 compensated summation (rk4)
@@ -298,10 +298,12 @@ using std::min;
 
 #include "../../ch_swe_amd/csrc/swe_sbp.hpp"
 #include "../../ch_swe_amd/csrc/swe_topology.hpp"
+#include "../../ch_swe_amd/csrc/swe_latlon.hpp"
 namespace {
 #include "../../ch_swe_amd/csrc/swe_geom.hpp"
 #include "../../ch_swe_amd/csrc/swe_fused.inc"
 #include "../../ch_swe_amd/csrc/swe_fused_host.hpp"
+#include "../../ch_swe_amd/csrc/swe_latlon.inc"
 }  // namespace
 
 // ---- host side ---------------------------------------------------------------------
@@ -490,6 +492,35 @@ struct Plan {
         for (int k = 0; k < 7; k++) out[k] = o[k];
     }
 
+    // Engine::latlon_h: the plan-time tables, the extended arrays, the
+    // interpolation (every buffer sized exactly)
+    void latlon(const T *st, int nlat, int nlon, float *out)
+    {
+        swe::LatlonTables t;
+        std::string err;
+        if (swe::build_latlon_tables(N, nlat, nlon, t, err)) {
+            std::fprintf(stderr, "%s\n", err.c_str());
+            std::exit(2);
+        }
+        const int n = N + 1;
+        std::vector<int> col(t.src.size());
+        for (size_t k = 0; k < col.size(); k++) {
+            const int s = t.src[k], p = s / (n * n), r = s - p * n * n, j = r / n + 1, i = r - (j - 1) * n + 1;
+            col[k] = (int)off(d, p, i, j);
+        }
+        const int nr = (int)t.rp.size() - 1, np = (int)t.base.size();
+        std::vector<double> ext(nr, NAN);
+        std::vector<float> o(np, NAN);
+        const T *h = st + 2 * d.F;
+        emu::launch_flat((nr + 255) / 256, 256, [&] {
+            k_ll_ext<T>(h, t.rp.data(), col.data(), t.w.data(), nr, ext.data());
+        });
+        emu::launch_flat((np + 255) / 256, 256, [&] {
+            k_ll_interp(ext.data(), t.base.data(), t.fx.data(), t.fy.data(), np, t.ew, o.data());
+        });
+        std::copy(o.begin(), o.end(), out);
+    }
+
     // Engine::integral_tends: apply into a tendency state, then the tile sums
     template <int O>
     void tends(T *st, T *tend, double grav, double out[2])
@@ -603,6 +634,27 @@ int run_case(const std::string &cs, int N, int steps, unsigned seed, int flags)
             std::printf("diag %s %.17g oracle %.17g rel err %.3e (bar %.1e)\n", nm[k], got[k], ref[k], e, dbar);
             bad |= !(e <= dbar);
         }
+    } else if (cs == "latlon") {
+        T *st = pl.state();
+        pl.upload(st, s0.u.data(), s0.v.data(), s0.h.data());
+        const int nlat = 2 * N + 1, nlon = 4 * N;
+        std::vector<float> got((size_t)nlat * nlon), ref((size_t)nlat * nlon);
+        pl.latlon(st, nlat, nlon, got.data());
+        std::vector<double> hh = s0.h;
+        if (sizeof(T) == 4)
+            for (double &x : hh) x = (double)(float)x;  // the oracle reads what the plan holds
+        swo_latlon_h(o, hh.data(), nlat, nlon, ref.data());
+        // float outputs of FP64 sums that differ in their last bits
+        int ulps = 0;
+        for (size_t k = 0; k < got.size(); k++) {
+            int32_t a, b;
+            std::memcpy(&a, &got[k], 4);
+            std::memcpy(&b, &ref[k], 4);
+            ulps = std::max(ulps, std::abs(a - b));
+            if (!std::isfinite(got[k])) ulps = 1 << 30;
+        }
+        std::printf("latlon %d x %d max ulp difference %d (bar 1)\n", nlat, nlon, ulps);
+        bad = ulps > 1;
     } else if (cs == "tends") {
         T *st = pl.state(), *tend = pl.state();
         Fields s1 = s0;
@@ -724,7 +776,7 @@ int dispatch(const std::string &cs, int N, int order, int steps, unsigned seed, 
 int main(int argc, char **argv)
 {
     if (argc < 5) {
-        std::fprintf(stderr, "usage: %s apply|rk4|integrals|diag|tends|multi N order 64|32 [steps] [seed] [flags]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s apply|rk4|integrals|diag|tends|latlon|multi N order 64|32 [steps] [seed] [flags]\n", argv[0]);
         return 2;
     }
     const std::string cs = argv[1];

@@ -7,7 +7,7 @@ blocks in a random order, every device buffer an exact-size heap allocation
 The cases cover single-tile panels, uneven last tiles, interior tiles, the
 general-tiles and plain-order flags, both SBP orders and precisions, the RK4
 stage modes (compensated summation included), the multi-instance (ensemble) kernels, the tile-wise
-integrals, the integral diagnostics and tendencies.  No GPU is used."""
+integrals, the integral diagnostics and tendencies and the lat-lon output.  No GPU is used."""
 import os
 import shutil
 import subprocess
@@ -41,6 +41,8 @@ CASES = [
     ("tends", 20, 42, 64, 1, 19, 0),    # integral tendencies (two passes over a tile's fields)
     ("tends", 32, 21, 64, 1, 20, 0),
     ("tends", 57, 42, 32, 1, 21, 0),
+    ("latlon", 12, 42, 64, 1, 22, 0),   # lat-lon output of h (ECS halo map, cubic interpolation)
+    ("latlon", 17, 21, 32, 1, 23, 0),
 ]
 
 
@@ -61,6 +63,7 @@ def emu_exe(tmp_path_factory):
     exe = str(out / "swe_emu")
     res = subprocess.run(["g++", "-std=c++17"] + common + ["-Wno-attributes", "-D__HIP_PLATFORM_AMD__",
                           "-I" + HIP_INCLUDE, EMU_SRC, os.path.join(ROOT, "ch_swe_amd", "csrc", "swe_topology.cpp"),
+                          os.path.join(ROOT, "ch_swe_amd", "csrc", "swe_latlon.cpp"),
                           obj, "-o", exe, "-lm"], capture_output=True, text=True)
     assert res.returncode == 0, res.stderr[-4000:]
     return exe

@@ -70,6 +70,20 @@ def test_gauss_config_matches_oracle(oracle_mod, tmp_path, order, precision, tol
     dtol = 1e-10 if precision.startswith("fp64") else 1e-4
     for k, g in zip(("hmin", "hmax", "mass", "TE", "KE", "PE"), vals):
         assert g == pytest.approx(d[k], rel=dtol, abs=dtol * abs(d["hmax"])), (k, g, d[k])
+    # h.dat: the initial state and the states after 1 and 2 hours, each
+    # within one float32 unit of the oracle's regrid of its RK4 state
+    hd = np.fromfile(tmp_path / "h.dat", dtype=np.float32)
+    assert hd.size == 3 * 65 * 128
+    hd = hd.reshape(3, 65, 128)
+    steps = [0, 4, 8]
+    for k, s in enumerate(steps):
+        hk = o.rk4(o.zeros(MESH_U), o.zeros(MESH_V), h0, 900.0, s)[2] if s else h0
+        if precision.startswith("fp32"):
+            continue  # an FP32 run's state differs from the oracle's at 1e-5
+        llref = o.latlon_h(hk)
+        diff = np.abs(hd[k].view(np.int32).astype(np.int64) - llref.view(np.int32).astype(np.int64))
+        scale = np.abs(llref).max()
+        assert np.all((diff <= 1) | (np.abs(hd[k] - llref) <= 1e-11 * scale)), (k, diff.max())
     # print_integral_tends's line after each diagnostics line (TE_tend only:
     # the enstrophy tendency divides by the perturbation h as well)
     tl = re.findall(r"dynamics: TE_tend = (.*), Enstrophy_tend = (.*)", stdout)

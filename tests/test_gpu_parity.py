@@ -395,6 +395,36 @@ def test_integrals_diag_matches_oracle(oracle_mod, order, N, precision, omega):
         assert abs(got[k] - r) <= tol * abs(r), (k, got[k], r)
 
 
+def _ulps(a, b):
+    return np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+
+
+LATLON_CASES = [(42, 12, cs.FP64, None), (21, 33, cs.FP64, None), (42, 96, cs.FP64, None), (42, 48, cs.FP32, None),
+                (21, 20, cs.FP64, (37, 50))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,N,precision,grid", LATLON_CASES)
+def test_latlon_h_matches_oracle(oracle_mod, order, N, precision, grid):
+    """swe_latlon_h (h.dat's record: the ECS-halo'd cubic regrid) against the
+    oracle on a random state: float32 outputs of FP64 sums, equal to within
+    one unit in the last place (an FP32 plan's h is compared with the
+    oracle's regrid of the same float values).  A second call on another
+    grid rebuilds the plan's tables."""
+    _say(f"latlon_h order={order} N={N} precision={precision} grid={grid}")
+    o = oracle_mod.Oracle(N, order)
+    u, v, h = _random_state(o, 80 + N)
+    op = _op(N, order, precision)
+    st = op.create_stvec().upload(u, v, h)
+    nlat, nlon = grid if grid else (2 * N + 1, 4 * N)
+    hh = h if precision == cs.FP64 else h.astype(np.float32).astype(np.float64)
+    for a, b in ((nlat, nlon), (nlat + 4, nlon + 3), (nlat, nlon)):
+        got = op.latlon_h(st, a, b)
+        ref = o.latlon_h(hh, a, b)
+        assert got.shape == (a, b)
+        assert _ulps(got, ref).max() <= 1, (a, b, _ulps(got, ref).max())
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("order,N,precision,omega", DIAG_CASES)
 def test_integral_tends_match_oracle(oracle_mod, order, N, precision, omega):
