@@ -38,7 +38,7 @@ EXPORTS = (
     "swe_last_error", "swe_version", "swe_plan_create", "swe_plan_destroy", "swe_plan_npts",
     "swe_plan_stream", "swe_synchronize", "swe_state_create", "swe_state_destroy",
     "swe_state_upload", "swe_state_download", "swe_apply", "swe_assign", "swe_update",
-    "swe_rk4_steps", "swe_integrals", "swe_integrals_noalloc", "swe_integrals_diag", "swe_integral_tends", "swe_latlon_h", "swe_halo_map", "swe_ensemble_create",
+    "swe_rk4_steps", "swe_integrals", "swe_integrals_noalloc", "swe_integrals_diag", "swe_integral_tends", "swe_cfl_diag", "swe_latlon_h", "swe_state_upload_halo", "swe_state_download_halo", "swe_halo_map", "swe_ensemble_create",
     "swe_ensemble_destroy", "swe_ensemble_rk4", "swe_ensemble_stream", "swe_ensemble_synchronize",
     "swe_plan_timing", "swe_plan_kernel_times", "swe_server_create", "swe_server_destroy",
     "swe_server_submit", "swe_server_tick", "swe_server_wait", "swe_server_take", "swe_server_stats",
@@ -108,6 +108,9 @@ def lib():
     L.swe_integrals_diag.argtypes = [_vp, _vp, ctypes.c_double, _dp]
     L.swe_integral_tends.argtypes = [_vp, _vp, ctypes.c_double, _dp]
     L.swe_latlon_h.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    L.swe_cfl_diag.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_double, _dp]
+    L.swe_state_upload_halo.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp]
+    L.swe_state_download_halo.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp]
     L.swe_halo_map.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip] + [_ip] * 9
     L.swe_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
     L.swe_ensemble_destroy.argtypes = [_vp]
@@ -238,6 +241,33 @@ class StvecSwm:
         _check(lib().swe_state_download(self.h, *[_p(a) for a in out]))
         return tuple(out)
 
+    def _halo_shapes(self, hw):
+        return {k: (6, a + 2 * hw, b + 2 * hw) for k, (_, a, b) in self.shapes.items()}
+
+    def upload_halo(self, hw, u, v, h):
+        """Upload from the reference's tile arrays with a halo of width hw
+        (p(1-hw:nx+hw, 1-hw:ny+hw) per panel); the halo entries are ignored."""
+        shp = self._halo_shapes(hw)
+        arrs = []
+        for name, a in (("u", u), ("v", v), ("h", h)):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.shape != shp[name]:
+                raise ValueError(f"{name}: expected {shp[name]}, got {a.shape}")
+            arrs.append(a)
+        _check(lib().swe_state_upload_halo(self.h, int(hw), *[_p(a) for a in arrs]))
+        return self
+
+    def download_halo(self, hw, u, v, h):
+        """Write the interior into the reference's halo'd tile arrays u, v, h
+        (float64, C-contiguous, shapes (6, ny + 2hw, nx + 2hw)); their halo
+        entries keep their values."""
+        shp = self._halo_shapes(hw)
+        for name, a in (("u", u), ("v", v), ("h", h)):
+            if a.shape != shp[name] or a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError(f"{name}: expected a C-contiguous float64 array of shape {shp[name]}")
+        _check(lib().swe_state_download_halo(self.h, int(hw), _p(u), _p(v), _p(h)))
+        return u, v, h
+
     # stvec_t%assign (s1*v1 [+ s2*v2]) and %update (+= s1*v1 [+ s2*v2])
     def assign(self, s1, v1, s2=0.0, v2=None):
         _check(lib().swe_assign(self._plan.h, self.h, s1, v1.h, s2, v2.h if v2 is not None else None))
@@ -283,6 +313,13 @@ class OperatorSwmLin:
         out = np.zeros(7)
         _check(lib().swe_integrals_diag(self.plan.h, state.h, float(grav), _p(out)))
         return dict(zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), out.tolist()))
+
+    def cfl_diag(self, state, dt, grav=9.80616):
+        """print_CFL_diag (swm_output_diag_mod.f90 L110-154): dict of the
+        gravity-wave and the two advective CFL numbers."""
+        out = np.zeros(3)
+        _check(lib().swe_cfl_diag(self.plan.h, state.h, float(dt), float(grav), _p(out)))
+        return dict(zip(("gravity", "advective_x", "advective_y"), out.tolist()))
 
     def latlon_h(self, state, nlat=None, nlon=None):
         """h on the lat-lon grid as write_fields writes h.dat

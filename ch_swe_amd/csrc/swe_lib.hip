@@ -390,6 +390,8 @@ struct EngineBase {
     virtual int integral_buffers(double **part, double **red) = 0;
     // print_integrals_diag's values: hmin, hmax, mass, TE, KE, PE, enstrophy
     virtual int integrals_diag(void *st, double grav, double out[7]) = 0;
+    // print_CFL_diag's values: gravity-wave CFL, advective CFL along x and y
+    virtual int cfl_diag(void *st, double dt, double grav, double out[3]) = 0;
     // print_integral_tends's values of the operator's tendency: TE_tend, Enstrophy_tend
     virtual int integral_tends(void *st, double grav, double out[2]) = 0;
     // h on the nlat x nlon lat-lon grid as the output writes h.dat (host out)
@@ -532,7 +534,7 @@ struct Engine : EngineBase {
         if ((rc = alloc((void **)&part, 3 * INT_MAX_BLOCKS * sizeof(double))) ||
             (rc = alloc((void **)&red, 4 * sizeof(double))) ||
             (rc = alloc((void **)&dpart, (size_t)DIAG_K * INT_MAX_BLOCKS * sizeof(double))) ||
-            (rc = alloc((void **)&dout, 8 * sizeof(double))))
+            (rc = alloc((void **)&dout, 10 * sizeof(double))))
             return rc;
         if (use_fused) rc = init_fused(geo);
         else rc = ensure_full();
@@ -796,7 +798,9 @@ struct Engine : EngineBase {
     void launch_diag(const T *st, double grav)
     {
         const int grid = std::min(ntiles, INT_MAX_BLOCKS);
-        k_diag<T, O, TY><<<grid, F_THREADS, 0, pl->stream>>>(d, cf, geo, st, ntx, nty, fwx, fwy, grav, dpart);
+        const double dx = geo.radius * geo.hx;
+        k_diag<T, O, TY><<<grid, F_THREADS, 0, pl->stream>>>(d, cf, geo, st, ntx, nty, fwx, fwy, grav, 1.0 / (dx * dx),
+                                                              dpart);
         k_diag_final<<<1, 64, 0, pl->stream>>>(dpart, grid, dout);
     }
     int integrals_diag(void *st, double grav, double out[7]) override
@@ -807,6 +811,23 @@ struct Engine : EngineBase {
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(out, dout, 7 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
         HIP_TRY(hipStreamSynchronize(pl->stream));
+        return SWE_OK;
+    }
+    int cfl_diag(void *st, double dt, double grav, double out[3]) override
+    {
+        if (!use_fused) return fail(SWE_EINVAL, "swe_cfl_diag: diagnostics run on fused plans only");
+        if (order == 21) launch_diag<21>((const T *)st, grav);
+        else launch_diag<42>((const T *)st, grav);
+        HIP_TRY(hipGetLastError());
+        double v[9];
+        HIP_TRY(hipMemcpyAsync(v, dout, 9 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        // sqrt of the largest g h / dx^2 (the largest h: the scaling is
+        // monotonic), of the largest |ut u| / dx^2 and |vt v| / dy^2, times dt
+        const double dx = geo.radius * geo.hx;
+        out[0] = std::sqrt(grav / (dx * dx) * v[1]) * dt;
+        out[1] = std::sqrt(v[7]) * dt;
+        out[2] = std::sqrt(v[8]) * dt;
         return SWE_OK;
     }
 
@@ -1032,6 +1053,25 @@ int download_t(swe_state *st, double *u, double *v, double *h)
     if (h) padded_to_pack(d, host.data() + 2 * d.F, N + 1, N + 1, h);
     return SWE_OK;
 }
+
+// the reference's tile arrays with a halo of width hw (grid_field_t tile
+// p(1-hw:nx+hw, 1-hw:ny+hw), one tile per panel, panels in order) <-> packed
+void halo_to_pack(const double *src, int nx, int ny, int hw, double *dst)
+{
+    const long sx = nx + 2 * hw, sy = ny + 2 * hw;
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+                dst[((long)p * ny + j - 1) * nx + i - 1] = src[(p * sy + j - 1 + hw) * sx + i - 1 + hw];
+}
+void pack_to_halo(const double *src, int nx, int ny, int hw, double *dst)
+{
+    const long sx = nx + 2 * hw, sy = ny + 2 * hw;
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+                dst[(p * sy + j - 1 + hw) * sx + i - 1 + hw] = src[((long)p * ny + j - 1) * nx + i - 1];
+}
 }  // namespace
 
 namespace swe {
@@ -1150,6 +1190,32 @@ int swe_state_download(swe_state *st, double *u, double *v, double *h)
     return st->plan->cfg.precision == SWE_FP64 ? download_t<double>(st, u, v, h) : download_t<float>(st, u, v, h);
 }
 
+int swe_state_upload_halo(swe_state *st, int hw, const double *u, const double *v, const double *h)
+{
+    if (!st || !u || !v || !h) return fail(SWE_EINVAL, "swe_state_upload_halo: NULL argument");
+    if (hw < 0 || hw > 64) return fail(SWE_EINVAL, "swe_state_upload_halo: halo width out of range 0..64");
+    const int N = st->plan->d.N;
+    std::vector<double> pu((size_t)6 * N * (N + 1)), pv(pu.size()), ph((size_t)6 * (N + 1) * (N + 1));
+    halo_to_pack(u, N, N + 1, hw, pu.data());
+    halo_to_pack(v, N + 1, N, hw, pv.data());
+    halo_to_pack(h, N + 1, N + 1, hw, ph.data());
+    return swe_state_upload(st, pu.data(), pv.data(), ph.data());
+}
+
+int swe_state_download_halo(swe_state *st, int hw, double *u, double *v, double *h)
+{
+    if (!st) return fail(SWE_EINVAL, "swe_state_download_halo: NULL state");
+    if (hw < 0 || hw > 64) return fail(SWE_EINVAL, "swe_state_download_halo: halo width out of range 0..64");
+    const int N = st->plan->d.N;
+    std::vector<double> pu((size_t)6 * N * (N + 1)), pv(pu.size()), ph((size_t)6 * (N + 1) * (N + 1));
+    int rc = swe_state_download(st, pu.data(), pv.data(), ph.data());
+    if (rc) return rc;
+    if (u) pack_to_halo(pu.data(), N, N + 1, hw, u);
+    if (v) pack_to_halo(pv.data(), N + 1, N, hw, v);
+    if (h) pack_to_halo(ph.data(), N + 1, N + 1, hw, h);
+    return SWE_OK;
+}
+
 int swe_apply(swe_plan *plan, swe_state *vout, swe_state *vin)
 {
     if (!plan) return fail(SWE_EINVAL, "NULL plan");
@@ -1257,6 +1323,15 @@ int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7]
     int rc = check_state(plan, st, "state");
     if (rc) return rc;
     return plan->eng->integrals_diag(st->buf, grav, out);
+}
+
+int swe_cfl_diag(swe_plan *plan, swe_state *st, double dt, double grav, double out[3])
+{
+    if (!plan || !out) return fail(SWE_EINVAL, "swe_cfl_diag: NULL argument");
+    if (!std::isfinite(grav) || !std::isfinite(dt)) return fail(SWE_EINVAL, "swe_cfl_diag: non-finite dt or gravity");
+    int rc = check_state(plan, st, "state");
+    if (rc) return rc;
+    return plan->eng->cfl_diag(st->buf, dt, grav, out);
 }
 
 int swe_latlon_h(swe_plan *plan, swe_state *st, int nlat, int nlon, float *out)

@@ -27,8 +27,9 @@ module swe_hip_capi_mod
     end type swe_config_t
 
     public :: swe_plan_create, swe_plan_destroy, swe_state_create, swe_state_destroy, &
-              swe_state_upload, swe_state_download, swe_apply, swe_assign, swe_update,  &
-              swe_rk4_steps, swe_integrals, swe_integrals_diag, swe_integral_tends, swe_latlon_h, &
+              swe_state_upload, swe_state_download, swe_state_upload_halo, swe_state_download_halo, &
+              swe_apply, swe_assign, swe_update,  &
+              swe_rk4_steps, swe_integrals, swe_integrals_diag, swe_integral_tends, swe_latlon_h, swe_cfl_diag, &
               swe_synchronize, swe_error_message
 
     interface
@@ -64,6 +65,19 @@ module swe_hip_capi_mod
             type(c_ptr), value :: st
             real(c_double), intent(out) :: u(*), v(*), h(*)
         end function
+        ! the reference's halo'd tile arrays p(1-hw:nx+hw, 1-hw:ny+hw), one tile per panel
+        integer(c_int) function swe_state_upload_halo(st, hw, u, v, h) bind(C, name="swe_state_upload_halo")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: st
+            integer(c_int), value :: hw
+            real(c_double), intent(in) :: u(*), v(*), h(*)
+        end function
+        integer(c_int) function swe_state_download_halo(st, hw, u, v, h) bind(C, name="swe_state_download_halo")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: st
+            integer(c_int), value :: hw
+            real(c_double), intent(inout) :: u(*), v(*), h(*)
+        end function
         integer(c_int) function swe_apply(plan, vout, vin) bind(C, name="swe_apply")
             import :: c_int, c_ptr
             type(c_ptr), value :: plan, vout, vin
@@ -94,6 +108,12 @@ module swe_hip_capi_mod
             type(c_ptr), value :: plan, st
             real(c_double), value :: grav
             real(c_double), intent(out) :: out(7)
+        end function
+        integer(c_int) function swe_cfl_diag(plan, st, dt, grav, out) bind(C, name="swe_cfl_diag")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: plan, st
+            real(c_double), value :: dt, grav
+            real(c_double), intent(out) :: out(3)
         end function
         integer(c_int) function swe_latlon_h(plan, st, nlat, nlon, out) bind(C, name="swe_latlon_h")
             import :: c_int, c_ptr, c_float

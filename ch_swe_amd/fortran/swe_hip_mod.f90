@@ -108,6 +108,7 @@ module swe_hip_mod
         procedure :: integrals_diag => operator_integrals_diag
         procedure :: integral_tends => operator_integral_tends
         procedure :: latlon_h => operator_latlon_h
+        procedure :: cfl_diag => operator_cfl_diag
     end type operator_swm_lin_hip_t
 
     type, extends(timescheme_t) :: rk4_hip_t
@@ -304,6 +305,17 @@ contains
         call check(swe_latlon_h(domain%plan, hip_state(v), int(size(hll, 2), c_int), int(size(hll, 1), c_int), hll), &
                    "latlon_h")
     end subroutine operator_latlon_h
+
+    ! swm_output_diag_mod.f90 print_CFL_diag L110-154: cfl = gravity-wave
+    ! CFL, advective CFL along x and y
+    subroutine operator_cfl_diag(this, v, domain, dt, grav, cfl)
+        class(operator_swm_lin_hip_t), intent(inout) :: this
+        class(stvec_t),                intent(in)    :: v
+        type(domain_hip_t),            intent(in)    :: domain
+        real(kind=8),                  intent(in)    :: dt, grav
+        real(kind=8),                  intent(out)   :: cfl(3)
+        call check(swe_cfl_diag(domain%plan, hip_state(v), dt, grav, cfl), "cfl_diag")
+    end subroutine operator_cfl_diag
 
     ! rk4_mod.f90 step_rk4 L20-42 (fused stages on the device)
     subroutine rk4_step(this, v0, operator, domain, dt)

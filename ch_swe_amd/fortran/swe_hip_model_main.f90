@@ -25,7 +25,8 @@
 !   diffusion_section absent or 'None'.
 ! g is Earth_grav (shallow_water_mod.f90 L93, const_mod.f90 L6).
 !
-! Output, as the reference prints it: the scores at t = 0 and at every
+! Output, as the reference prints it: print_CFL_diag's line
+! (swm_output_diag_mod.f90 L110-154, swe_cfl_diag) first, the scores at t = 0 and at every
 ! diagnostics step (Gaussian hill: against the reference solution file
 ! refsol_path when it exists, gaussian_hill_linear_test_mod.f90 L71-129;
 ! TS2: relative l2 / linf error of h, Williamson_test2_mod.f90 L115-143),
@@ -68,7 +69,7 @@ program swe_hip_model
     integer :: N, order, precision, flags, nstep_write, nstep_diagnostics, nsteps, it, nrun, p, i, j, k
     real(kind=8) :: dt, tau_write, tau_diagnostics, simulation_time, radius, omega, h_mean, axis(3), rm(9)
     real(kind=8) :: lam0, phi0, d0, u0, tc_h_mean, an(3), c(3), r(3), al, be, X, Y, s, vel(3), a1(3), a2(3)
-    real(kind=8) :: diag(7), tends(2), l2_h0, linf_h0
+    real(kind=8) :: diag(7), tends(2), cfl(3), l2_h0, linf_h0
     real(kind=4), allocatable :: hll(:, :)
     logical :: coriolis_const, linear_mode, gauss, have_refsol
 
@@ -241,8 +242,11 @@ program swe_hip_model
     call create_stvec_hip(state, domain)
     call state%upload(u, v, h)
 
-    ! write_fields (swm_output_diag_mod.f90 L52-73): h.dat record 1 holds the
-    ! initial state, record k + 1 the state after k write intervals
+    ! print_CFL_diag (swm_output_diag_mod.f90 L110-154), then write_fields
+    ! (L52-73): h.dat record 1 holds the initial state, record k + 1 the
+    ! state after k write intervals
+    call operator%cfl_diag(state, domain, dt, Earth_grav, cfl)
+    print '(A,F15.7,A,2F15.7)', "Gravity waves CFL-max = ", cfl(1), ", Advective CFL-max (x,y) = ", cfl(2), cfl(3)
     allocate(hll(4 * N, 2 * N + 1))
     call write_h(1)
     call print_scores(0)

@@ -92,6 +92,17 @@ int swe_state_create(swe_plan *plan, swe_state **out);
 int swe_state_destroy(swe_state *st);
 int swe_state_upload(swe_state *st, const double *u, const double *v, const double *h);
 int swe_state_download(swe_state *st, double *u, double *v, double *h);
+/* The same exchange with the reference's own tile arrays, halos included:
+ * grid_field_t%tile(t)%p(is-hw:ie+hw, js-hw:je+hw, 1:1)
+ * (grid_field_factory_mod.f90 create_grid_field L9-31, grid_field_mod.f90
+ * tile_field_init L78-95) for the
+ * partition of one tile per panel, the six tiles back to back (u: (N+2hw) x
+ * (N+1+2hw) per panel, v: (N+1+2hw) x (N+2hw), h: (N+1+2hw)^2, column-major).
+ * Upload reads only the interior; download writes only the interior and
+ * leaves the halo entries as they were (the reference fills halos by
+ * exchange when an operator needs them).  0 <= hw <= 64. */
+int swe_state_upload_halo(swe_state *st, int hw, const double *u, const double *v, const double *h);
+int swe_state_download_halo(swe_state *st, int hw, double *u, double *v, double *h);
 
 /* operator_t%apply (src/operator_mod.f90 L11-41) for operator_swm_lin_t
  * (operator_swm_lin_mod.f90 apply L66-108): vout = A(vin). */
@@ -133,6 +144,15 @@ int swe_integrals_noalloc(swe_plan *plan, swe_state *st, double *mass, double *e
  * grav is the PE's gravity (the reference uses Earth_grav there whatever the
  * operator's).  Fused plans only; blocks until the values are on the host. */
 int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7]);
+
+/* swm_output_diag_t%print_CFL_diag
+ * (models/shallow_water/output_and_diag/swm_output_diag_mod.f90 L110-154),
+ * covariant winds: out[3] = the gravity-wave CFL sqrt(max g h / dx^2) dt
+ * and the advective CFLs sqrt(max |ut u| / dx^2) dt, sqrt(max |vt v| /
+ * dy^2) dt (ut, vt contravariant; dx = dy = radius * pi / (2N)); grav as the
+ * reference passes it there (Earth_grav).  Fused plans only; blocks until
+ * the values are on the host. */
+int swe_cfl_diag(swe_plan *plan, swe_state *st, double dt, double grav, double out[3]);
 
 /* The lat-lon output of h: swm_output_diag_t%write_fields's h.dat
  * (models/shallow_water/output_and_diag/swm_output_diag_mod.f90 L52-73,

@@ -1191,6 +1191,27 @@ void swo_integrals_diag(swo *s, const double *u, const double *v, const double *
     out[6] = ens;
 }
 
+/* print_CFL_diag (swm_output_diag_mod.f90 L110-154), covariant winds: the
+ * gravity-wave CFL sqrt(max(g / dx^2 h)) dt and the advective ones
+ * sqrt(max |(ut / dx^2) u|) dt, sqrt(max |(vt / dy^2) v|) dt, ut and vt the
+ * contravariant components (co2contra), dx = dy = a pi / (2N). */
+void swo_cfl_diag(swo *s, const double *u, const double *v, const double *h, double dt, double grav, double out[3])
+{
+    const int nu = swo_npts(s, SWO_MESH_U), nv = swo_npts(s, SWO_MESH_V), nh = swo_npts(s, SWO_MESH_XY);
+    double *uc = malloc(nu * sizeof(double)), *vc = malloc(nv * sizeof(double));
+    swo_co2contra(s, u, v, uc, vc);
+    const double dx = s->a * s->hx, cg = grav / (dx * dx), c = 1.0 / (dx * dx);
+    double mg = -INFINITY, mu = 0, mv = 0;
+    for (int k = 0; k < nh; k++) mg = fmax(mg, cg * h[k]);
+    for (int k = 0; k < nu; k++) mu = fmax(mu, fabs(c * uc[k] * u[k]));
+    for (int k = 0; k < nv; k++) mv = fmax(mv, fabs(c * vc[k] * v[k]));
+    out[0] = sqrt(mg) * dt;
+    out[1] = sqrt(mu) * dt;
+    out[2] = sqrt(mv) * dt;
+    free(uc);
+    free(vc);
+}
+
 /* print_integral_tends (swm_output_diag_mod.f90 L236-342) of the linear
  * operator's tendency, Ch with covariant winds, no orography: out[0] =
  * TE_tend = sum_u w (I_x(h_t) ut u / 2 + I_x(h) ut u_t) + (the same on v) +

@@ -72,6 +72,8 @@ void swo_integrals_diag(swo *s, const double *u, const double *v, const double *
 /* h (packed xy layout) on the nlat x nlon lat-lon grid as the shallow-water
  * output writes h.dat: float, longitude fastest */
 void swo_latlon_h(swo *s, const double *h, int nlat, int nlon, float *out);
+/* print_CFL_diag: gravity-wave CFL, advective CFL along x and y */
+void swo_cfl_diag(swo *s, const double *u, const double *v, const double *h, double dt, double grav, double out[3]);
 /* print_integral_tends of the operator's tendency: out = TE_tend,
  * Enstrophy_tend, then the sums of their terms' absolute values */
 void swo_integral_tends(swo *s, const double *u, const double *v, const double *h, double grav, double out[4]);

@@ -54,6 +54,7 @@ def _load():
     lib.swo_curl.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     lib.swo_integrals_diag.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
     lib.swo_integral_tends.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
+    lib.swo_cfl_diag.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
     lib.swo_latlon_h.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     lib.swo_gaussian_hill.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]
     lib.swo_ts2_linear.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
@@ -223,6 +224,13 @@ class Oracle:
         out = np.zeros(4)
         self.lib.swo_integral_tends(self._h, _p(u), _p(v), _p(h), grav, _p(out))
         return tuple(float(x) for x in (out if scales else out[:2]))
+
+    def cfl_diag(self, u, v, h, dt, grav=9.80616):
+        """print_CFL_diag: (gravity-wave CFL, advective CFL x, y)."""
+        u, v, h = self._c(u), self._c(v), self._c(h)
+        out = np.zeros(3)
+        self.lib.swo_cfl_diag(self._h, _p(u), _p(v), _p(h), dt, grav, _p(out))
+        return tuple(float(x) for x in out)
 
     def latlon_h(self, h, nlat=None, nlon=None):
         """h on the lat-lon grid as the shallow-water output writes h.dat

@@ -485,9 +485,11 @@ struct Plan {
     void diag(const T *st, double grav, double out[7])
     {
         const int grid = std::min(ft.ntiles, INT_MAX_BLOCKS);
-        std::vector<double> prt(DIAG_K * (size_t)grid, NAN), o(8, NAN);
+        std::vector<double> prt(DIAG_K * (size_t)grid, NAN), o(9, NAN);
         double *pp = prt.data(), *op = o.data();
-        emu::launch(grid, F_THREADS, [&] { k_diag<T, O, TY>(d, cf, g, st, ft.ntx, ft.nty, ft.fwx, ft.fwy, grav, pp); });
+        const double dx = RADIUS * g.hx;
+        emu::launch(grid, F_THREADS,
+                    [&] { k_diag<T, O, TY>(d, cf, g, st, ft.ntx, ft.nty, ft.fwx, ft.fwy, grav, 1.0 / (dx * dx), pp); });
         emu::launch_flat(1, 64, [&] { k_diag_final(pp, grid, op); });
         for (int k = 0; k < 7; k++) out[k] = o[k];
     }

@@ -122,6 +122,11 @@ def test_ts2_config_matches_oracle(oracle_mod, tmp_path, order):
     te, en, ste, sen = o.integral_tends(*ref, EARTH_GRAV, scales=True)
     assert abs(float(tl[-1][0]) - te) <= 1e-10 * ste + 1e-11 * abs(te), (tl[-1], te, ste)
     assert abs(float(tl[-1][1]) - en) <= 1e-10 * sen + 1e-11 * abs(en), (tl[-1], en, sen)
+    # print_CFL_diag's line of the initial state ('(A,F15.7,A,2F15.7)')
+    cl = re.findall(r"Gravity waves CFL-max =\s*(\S+), Advective CFL-max \(x,y\) =\s*(\S+)\s+(\S+)", stdout)
+    assert len(cl) == 1
+    for g, r in zip(cl[0], o.cfl_diag(*init, 900.0, EARTH_GRAV)):
+        assert abs(float(g) - r) <= 6e-8, (cl, r)
     errs = re.findall(r"Errors, t=\s*(\S+)\s+hours, l2_h =\s*(\S+)\s*, linf_h =\s*(\S+)", stdout)
     assert [float(e[0]) for e in errs] == [0.0, 1.0, 2.0]
     assert float(errs[0][1]) == 0.0 and 0 < float(errs[2][1]) < 1e-3

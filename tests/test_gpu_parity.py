@@ -393,6 +393,41 @@ def test_integrals_diag_matches_oracle(oracle_mod, order, N, precision, omega):
     tol = 1e-12 if precision == cs.FP64 else 2e-6
     for k, r in ref.items():
         assert abs(got[k] - r) <= tol * abs(r), (k, got[k], r)
+    # print_CFL_diag from the same launch's maxima
+    gc = op.cfl_diag(st, 600.0, 9.80616)
+    rc = o.cfl_diag(u, v, h, 600.0, 9.80616)
+    for k, r in zip(("gravity", "advective_x", "advective_y"), rc):
+        assert abs(gc[k] - r) <= tol * abs(r), (k, gc[k], r)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hw,precision", [(0, cs.FP64), (2, cs.FP64), (8, cs.FP32)])
+def test_halo_tile_array_exchange(hw, precision):
+    """swe_state_upload_halo / download_halo with the reference's halo'd
+    tile arrays: the interior goes in and out as through the packed entry
+    points, halo entries are ignored on upload and kept on download."""
+    _say(f"halo tile arrays hw={hw} precision={precision}")
+    N = 13
+    op = _op(N, 42, precision)
+    rng = np.random.default_rng(hw)
+    shp = {"u": (6, N + 1, N), "v": (6, N, N + 1), "h": (6, N + 1, N + 1)}
+    packed = {k: rng.standard_normal(s) for k, s in shp.items()}
+    halo = {k: rng.standard_normal((6, s[1] + 2 * hw, s[2] + 2 * hw)) for k, s in shp.items()}
+    for k, s in shp.items():
+        halo[k][:, hw:hw + s[1], hw:hw + s[2]] = packed[k]
+    st = op.create_stvec().upload_halo(hw, halo["u"], halo["v"], halo["h"])
+    ref = op.create_stvec().upload(packed["u"], packed["v"], packed["h"])
+    for a, b in zip(st.download(), ref.download()):
+        assert np.array_equal(a, b)
+    out = {k: np.full_like(a, 7.0) for k, a in halo.items()}
+    st.download_halo(hw, out["u"], out["v"], out["h"])
+    got = ref.download()
+    for (k, s), g in zip(shp.items(), got):
+        inner = out[k][:, hw:hw + s[1], hw:hw + s[2]]
+        assert np.array_equal(inner, g)
+        mask = np.ones(out[k].shape, bool)
+        mask[:, hw:hw + s[1], hw:hw + s[2]] = False
+        assert np.all(out[k][mask] == 7.0)
 
 
 def _ulps(a, b):

@@ -230,3 +230,42 @@ def test_tends_at_rest_and_scales(oracle_mod):
     u, v, h = _tends_state(o, 14, 30.0)
     te, en, ste, sen = o.integral_tends(u, v, h, scales=True)
     assert abs(te) <= ste and abs(en) <= sen and ste > 0 and sen > 0
+
+
+def test_cfl_at_rest(oracle_mod):
+    """print_CFL_diag at rest on a layer of depth c: the gravity-wave CFL is
+    sqrt(g c) dt / dx with dx = a pi / (2N), the advective ones vanish."""
+    N, c, dt, g = 20, 800.0, 600.0, 9.80616
+    o = oracle_mod.Oracle(N, 42, radius=A)
+    cg, cx, cy = o.cfl_diag(o.zeros(MESH_U), o.zeros(MESH_V), np.full(o.shape(MESH_XY), c), dt, g)
+    assert cg == pytest.approx(np.sqrt(g * c) * dt / (A * np.pi / (2 * N)), rel=1e-14)
+    assert cx == 0.0 and cy == 0.0
+
+
+@pytest.mark.parametrize("order,rate", [(21, 0.9), (42, 1.8)])
+def test_cfl_solid_rotation_converges(oracle_mod, order, rate):
+    """Solid rotation about a tilted axis: the advective CFLs converge to
+    sqrt(max |V.a^1 V.a_1|) dt / dx with the exact contravariant component
+    V.a^1 = g^11 V.a_1 + g^12 V.a_2 from the basis vectors (the oracle's
+    co2contra is the SBP approximation of it)."""
+    axis, u0, dt = (0.2, 0.4, 0.9), 40.0, 300.0
+    errs = []
+    for N in (16, 32, 64):
+        o = oracle_mod.Oracle(N, order, radius=A)
+        u, v = _solid_rotation(o, u0, axis)
+        exact = []
+        for m, cov in ((MESH_U, u), (MESH_V, v)):
+            a1, a2 = o.basis(m)
+            g11, g12, g22 = (np.einsum("...k,...k->...", a, b) for a, b in ((a1, a1), (a1, a2), (a2, a2)))
+            det = g11 * g22 - g12 * g12
+            ax = np.array(axis) / np.linalg.norm(axis)
+            r = np.cross(a1, a2)
+            r /= np.linalg.norm(r, axis=-1, keepdims=True)
+            V = u0 * np.cross(ax, r)
+            c1, c2 = np.einsum("...k,...k->...", V, a1), np.einsum("...k,...k->...", V, a2)
+            contra = (g22 * c1 - g12 * c2) / det if m == MESH_U else (-g12 * c1 + g11 * c2) / det
+            own = c1 if m == MESH_U else c2
+            exact.append(np.sqrt(np.abs(contra * own).max()) * dt / (A * np.pi / (2 * N)))
+        got = o.cfl_diag(u, v, np.full(o.shape(MESH_XY), 100.0), dt)[1:]
+        errs.append(max(abs(got[k] / exact[k] - 1) for k in range(2)))
+    assert errs[-1] < 1e-3 and np.all(_conv(errs) > rate), errs
