@@ -38,7 +38,8 @@ EXPORTS = (
     "swe_last_error", "swe_version", "swe_plan_create", "swe_plan_destroy", "swe_plan_npts",
     "swe_plan_stream", "swe_synchronize", "swe_state_create", "swe_state_destroy",
     "swe_state_upload", "swe_state_download", "swe_apply", "swe_assign", "swe_update",
-    "swe_rk4_steps", "swe_integrals", "swe_integrals_noalloc", "swe_integrals_diag", "swe_integral_tends", "swe_cfl_diag", "swe_latlon_h", "swe_state_upload_halo", "swe_state_download_halo", "swe_halo_map", "swe_ensemble_create",
+    "swe_rk4_steps", "swe_integrals", "swe_integrals_noalloc", "swe_integrals_diag", "swe_integral_tends", "swe_cfl_diag", "swe_latlon_h", "swe_solve", "swe_implicit_euler_steps",
+    "swe_crank_nicolson_steps", "swe_state_upload_halo", "swe_state_download_halo", "swe_halo_map", "swe_ensemble_create",
     "swe_ensemble_destroy", "swe_ensemble_rk4", "swe_ensemble_stream", "swe_ensemble_synchronize",
     "swe_plan_timing", "swe_plan_kernel_times", "swe_server_create", "swe_server_destroy",
     "swe_server_submit", "swe_server_tick", "swe_server_wait", "swe_server_take", "swe_server_stats",
@@ -109,6 +110,9 @@ def lib():
     L.swe_integral_tends.argtypes = [_vp, _vp, ctypes.c_double, _dp]
     L.swe_latlon_h.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     L.swe_cfl_diag.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_double, _dp]
+    L.swe_solve.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, _ip, _dp]
+    for name in ("swe_implicit_euler_steps", "swe_crank_nicolson_steps"):
+        getattr(L, name).argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, _dp]
     L.swe_state_upload_halo.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp]
     L.swe_state_download_halo.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp]
     L.swe_halo_map.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip] + [_ip] * 9
@@ -313,6 +317,16 @@ class OperatorSwmLin:
         out = np.zeros(7)
         _check(lib().swe_integrals_diag(self.plan.h, state.h, float(grav), _p(out)))
         return dict(zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), out.tolist()))
+
+    def solve(self, vout, rhs, dt, rel_tol=1e-10, max_iter=200):
+        """operator_swm_lin_t%solve (operator_swm_lin_mod.f90 L110-146):
+        vout = (I - dt L0)^-1 rhs without the Coriolis term, CG on the
+        Helmholtz problem for h (helm_solver_cfg: rel_tol, max_iter).
+        Returns (iterations, final relative residual)."""
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        _check(lib().swe_solve(self.plan.h, vout.h, rhs.h, float(dt), int(max_iter), float(rel_tol),
+                               ctypes.byref(it), ctypes.byref(rr)))
+        return it.value, rr.value
 
     def cfl_diag(self, state, dt, grav=9.80616):
         """print_CFL_diag (swm_output_diag_mod.f90 L110-154): dict of the
@@ -576,11 +590,34 @@ def parse_config_file(path):
     return GenericConfig(path=path)
 
 
-def create_timescheme(name):
-    """Factory mirroring timescheme_factory_mod.f90 (only 'rk4' is on this path)."""
-    if name != "rk4":
-        raise ValueError(f"unknown time scheme '{name}' (this build provides 'rk4')")
-    return Rk4()
+class ImplicitScheme:
+    """implicit_Eul_t (time_schemes/implicit_Eul_mod.f90 L20-30) or
+    crank_nikolson_t (crank_nikolson_mod.f90 L20-32) on the operator's solve;
+    rel_tol / max_iter: the helm_solver_cfg of the operator.  After a step,
+    .iters and .relres hold the largest CG iteration count and final
+    relative residual over its steps."""
+
+    def __init__(self, name, rel_tol=1e-10, max_iter=200):
+        self.name, self.rel_tol, self.max_iter = name, float(rel_tol), int(max_iter)
+        self._fn = "swe_crank_nicolson_steps" if name == "crank_nikolson" else "swe_implicit_euler_steps"
+        self.iters, self.relres = 0, 0.0
+
+    def step(self, v0, operator, dt, nsteps=1):
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        _check(getattr(lib(), self._fn)(operator.plan.h, v0.h, float(dt), int(nsteps), self.max_iter, self.rel_tol,
+                                        ctypes.byref(it), ctypes.byref(rr)))
+        self.iters, self.relres = it.value, rr.value
+
+
+def create_timescheme(name, rel_tol=1e-10, max_iter=200):
+    """Factory mirroring timescheme_factory_mod.f90 L28-45 for the schemes on
+    this path: 'rk4', 'implicit_Eul', 'crank_nikolson' (the last two take the
+    operator's Helmholtz solver settings)."""
+    if name == "rk4":
+        return Rk4()
+    if name in ("implicit_Eul", "crank_nikolson"):
+        return ImplicitScheme(name, rel_tol, max_iter)
+    raise ValueError(f"unknown time scheme '{name}' (this build provides rk4, implicit_Eul, crank_nikolson)")
 
 
 def check_tables(N, precision=FP32):

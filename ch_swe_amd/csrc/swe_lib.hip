@@ -254,6 +254,60 @@ __global__ void k_axpy(long n, T *__restrict__ out, T s1, const T *__restrict__ 
     out[k] = update ? out[k] + r : r;
 }
 
+// ---- the implicit solve's conjugate gradients ---------------------------------
+// cg_solver_mod.f90 L19-90 on the h field with the xy quadrature's inner
+// product (grid_field_based_vector_mod.f90 dot / norm L169-191).
+
+// sum over the interior xy points of w a b (w: the SBP_C quadrature weight),
+// FP64 partials per block in a fixed order
+template <typename T, int O>
+__global__ __launch_bounds__(256) void k_xy_dot(Dims d, double hx, const T *a, const T *b, double *part)
+{
+    __shared__ double ws[4];
+    const int n = d.N + 1;
+    const long np = 6L * n * n;
+    double s = 0;
+    for (long k = blockIdx.x * 256L + threadIdx.x; k < np; k += gridDim.x * 256L) {
+        int p, i, j;
+        unflat(k, n, n, p, i, j);
+        const long o = off(d, p, i, j);
+        s += quad_weight(d.N, O, hx, 0, i, j) * (double)a[o] * (double)b[o];
+    }
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_down(s, m);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+__global__ void k_dot_final(const double *part, int nb, double *out)
+{
+    if (threadIdx.x != 0) return;
+    double s = 0;
+    for (int b = 0; b < nb; b++) s += part[b];
+    *out = s;
+}
+
+// x += alpha p, r -= alpha Ap, alpha = |r|^2 / (p, Ap) with |r| = sqrt(*rr)
+template <typename T>
+__global__ void k_cg_step(long n, T *x, T *r, const T *p, const T *Ap, const double *rr, const double *pap)
+{
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double rn = sqrt(*rr), alpha = rn * rn / *pap;
+    x[k] = x[k] + T(alpha) * p[k];
+    r[k] = r[k] + T(-alpha) * Ap[k];
+}
+
+// p = r + beta p, beta = (|r_new| / |r_old|)^2
+template <typename T>
+__global__ void k_cg_dir(long n, T *p, const T *r, const double *rr_new, const double *rr_old)
+{
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double q = sqrt(*rr_new) / sqrt(*rr_old), beta = q * q;
+    p[k] = T(1.0) * r[k] + T(beta) * p[k];
+}
+
 // FP64 reduction over interior points: per-block partials in a fixed order
 template <typename T>
 __global__ void k_reduce(Dims d, const T *h, const T *u, const T *v, const T *ut, const T *vt, const double *wxy,
@@ -390,6 +444,8 @@ struct EngineBase {
     virtual int integral_buffers(double **part, double **red) = 0;
     // print_integrals_diag's values: hmin, hmax, mass, TE, KE, PE, enstrophy
     virtual int integrals_diag(void *st, double grav, double out[7]) = 0;
+    // operator_swm_lin_t%solve: vout = (I - dt L0)^-1 rhs (L0: no Coriolis)
+    virtual int solve(void *vout, void *rhs, double dt, int max_iter, double rel_tol, int *iters, double *relres) = 0;
     // print_CFL_diag's values: gravity-wave CFL, advective CFL along x and y
     virtual int cfl_diag(void *st, double dt, double grav, double out[3]) = 0;
     // print_integral_tends's values of the operator's tendency: TE_tend, Enstrophy_tend
@@ -416,6 +472,7 @@ struct swe_plan {
     hipStream_t stream = nullptr;
     std::unique_ptr<EngineBase> eng;
     swe_state *rk[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // k1..k4, y
+    swe_state *imp = nullptr;  // v_new of the implicit Euler and Crank-Nicolson steps
 };
 
 struct swe_state {
@@ -894,6 +951,90 @@ struct Engine : EngineBase {
         return SWE_OK;
     }
 
+    // the implicit solve: two apply states and their results, the CG
+    // vectors r, p, Ap, b (h fields), dot partials and scalars (allocated on
+    // the first call)
+    T *sv_s = nullptr, *sv_t1 = nullptr, *sv_t2 = nullptr, *sv_r = nullptr, *sv_p = nullptr, *sv_ap = nullptr,
+      *sv_b = nullptr;
+    double *sv_part = nullptr, *sv_sc = nullptr;
+    static constexpr int DOT_BLOCKS = 256;
+    void xy_dot(const T *a, const T *b, double *out)
+    {
+        if (order == 21) k_xy_dot<T, 21><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, geo.hx, a, b, sv_part);
+        else k_xy_dot<T, 42><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, geo.hx, a, b, sv_part);
+        k_dot_final<<<1, 64, 0, pl->stream>>>(sv_part, DOT_BLOCKS, out);
+    }
+    // swm_helm_lin_operator_t%apply: az = z - gamma div co2contra grad z with
+    // gamma = H g dt^2, through two applies of the linear operator: apply(0,
+    // 0, z) has u, v = -g grad z, the next apply's h = -H div co2contra of
+    // those = H g div co2contra grad z
+    int helm(const T *z, T *az, double dt)
+    {
+        int rc;
+        HIP_TRY(hipMemcpyAsync(sv_s + 2 * d.F, z, d.F * sizeof(T), hipMemcpyDeviceToDevice, pl->stream));
+        if ((rc = apply(sv_t1, sv_s)) || (rc = apply(sv_t2, sv_t1))) return rc;
+        k_axpy<T><<<blocks_for(d.F), TPB, 0, pl->stream>>>(d.F, az, T(1.0), z, T(-(dt * dt)), sv_t2 + 2 * d.F, 0);
+        return SWE_OK;
+    }
+    int solve(void *vout, void *rhs, double dt, int max_iter, double rel_tol, int *iters, double *relres) override
+    {
+        int rc;
+        if (!sv_s) {
+            T **st3[] = {&sv_s, &sv_t1, &sv_t2};
+            for (T **q : st3)
+                if ((rc = alloc((void **)q, 3 * d.F * sizeof(T)))) return rc;
+            T **f1[] = {&sv_r, &sv_p, &sv_ap, &sv_b};
+            for (T **q : f1)
+                if ((rc = field(q))) return rc;
+            if ((rc = alloc((void **)&sv_part, DOT_BLOCKS * sizeof(double))) ||
+                (rc = alloc((void **)&sv_sc, 8 * sizeof(double))))
+                return rc;
+        }
+        T *out = (T *)vout, *in = (T *)rhs, *x = out + 2 * d.F;
+        const long F = d.F;
+        // Helmholtz right-hand side b = rhs_h - H dt div co2contra(rhs_u, rhs_v)
+        if ((rc = apply(sv_t1, in))) return rc;
+        k_axpy<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, sv_b, T(1.0), in + 2 * F, T(dt), sv_t1 + 2 * F, 0);
+        HIP_TRY(hipMemcpyAsync(x, sv_b, F * sizeof(T), hipMemcpyDeviceToDevice, pl->stream));
+        if ((rc = helm(x, sv_ap, dt))) return rc;
+        k_axpy<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, sv_r, T(-1.0), sv_ap, T(1.0), sv_b, 0);
+        xy_dot(sv_r, sv_r, sv_sc + 0);
+        xy_dot(sv_b, sv_b, sv_sc + 3);
+        double h2[4];
+        HIP_TRY(hipMemcpyAsync(h2, sv_sc, 4 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        const double bn = std::sqrt(h2[3]);
+        // a zero right-hand side has the solution zero (x = b = 0 already;
+        // the reference's 0 / 0 would go on with NaN)
+        double rel = bn == 0.0 ? 0.0 : std::sqrt(h2[0]) / bn;
+        int it = 0;
+        if (!(rel < rel_tol)) {
+            HIP_TRY(hipMemcpyAsync(sv_p, sv_r, F * sizeof(T), hipMemcpyDeviceToDevice, pl->stream));
+            int a = 0, b = 2;  // slots of |r|^2 before and after the step
+            for (it = 1; it <= max_iter; it++) {
+                if ((rc = helm(sv_p, sv_ap, dt))) return rc;
+                xy_dot(sv_p, sv_ap, sv_sc + 1);
+                k_cg_step<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, x, sv_r, sv_p, sv_ap, sv_sc + a, sv_sc + 1);
+                xy_dot(sv_r, sv_r, sv_sc + b);
+                double rrn;
+                HIP_TRY(hipMemcpyAsync(&rrn, sv_sc + b, sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+                HIP_TRY(hipStreamSynchronize(pl->stream));
+                rel = std::sqrt(rrn) / bn;
+                if (rel < rel_tol) break;
+                k_cg_dir<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, sv_p, sv_r, sv_sc + b, sv_sc + a);
+                std::swap(a, b);
+            }
+        }
+        if (iters) *iters = it;
+        if (relres) *relres = rel;
+        // u, v = rhs_u, rhs_v - g dt grad h
+        HIP_TRY(hipMemcpyAsync(sv_s + 2 * F, x, F * sizeof(T), hipMemcpyDeviceToDevice, pl->stream));
+        if ((rc = apply(sv_t1, sv_s))) return rc;
+        k_axpy<T><<<blocks_for(2 * F), TPB, 0, pl->stream>>>(2 * F, out, T(1.0), in, T(dt), sv_t1, 0);
+        HIP_TRY(hipGetLastError());
+        return SWE_OK;
+    }
+
     T *tbuf = nullptr;  // the tendency of swe_integral_tends (allocated on its first call)
     template <int O>
     void launch_tends(const T *st, double grav)
@@ -1130,6 +1271,8 @@ int swe_plan_destroy(swe_plan *plan)
         swe_state_destroy(s);
         s = nullptr;
     }
+    swe_state_destroy(plan->imp);
+    plan->imp = nullptr;
     plan->eng.reset();
     if (plan->stream) (void)hipStreamDestroy(plan->stream);
     delete plan;
@@ -1323,6 +1466,64 @@ int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7]
     int rc = check_state(plan, st, "state");
     if (rc) return rc;
     return plan->eng->integrals_diag(st->buf, grav, out);
+}
+
+int swe_solve(swe_plan *plan, swe_state *vout, swe_state *rhs, double dt, int max_iter, double rel_tol, int *iters,
+              double *relres)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    int rc;
+    if ((rc = check_state(plan, vout, "vout")) || (rc = check_state(plan, rhs, "rhs"))) return rc;
+    if (vout == rhs) return fail(SWE_EINVAL, "swe_solve: vout and rhs must differ");
+    if (!std::isfinite(dt) || !(rel_tol > 0) || max_iter < 0)
+        return fail(SWE_EINVAL, "swe_solve: need a finite dt, rel_tol > 0 and max_iter >= 0");
+    return plan->eng->solve(vout->buf, rhs->buf, dt, max_iter, rel_tol, iters, relres);
+}
+
+// implicit_Eul_mod.f90 step_implicit_Eul L20-30 / crank_nikolson_mod.f90
+// step_crank_nikolson L20-32, nsteps times; *iters, *relres: the largest
+// CG iteration count and final relative residual over the steps
+static int implicit_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps, int max_iter, double rel_tol,
+                          int *iters, double *relres, bool cn)
+{
+    if (!plan) return fail(SWE_EINVAL, "NULL plan");
+    int rc = check_state(plan, v0, "v0");
+    if (rc) return rc;
+    if (nsteps < 0) return fail(SWE_EINVAL, "nsteps < 0");
+    if (!plan->imp && (rc = swe_state_create(plan, &plan->imp))) return rc;
+    swe_state *vn = plan->imp;
+    int itmax = 0;
+    double rmax = 0;
+    for (int k = 0; k < nsteps; k++) {
+        int it = 0;
+        double rr = 0;
+        if (cn) {
+            if ((rc = swe_apply(plan, vn, v0)) || (rc = swe_assign(plan, vn, 0.5 * dt, vn, 1.0, v0)) ||
+                (rc = swe_solve(plan, v0, vn, 0.5 * dt, max_iter, rel_tol, &it, &rr)))
+                return rc;
+        } else {
+            if ((rc = swe_solve(plan, vn, v0, dt, max_iter, rel_tol, &it, &rr)) ||
+                (rc = swe_assign(plan, v0, 1.0, vn, 0.0, nullptr)))
+                return rc;
+        }
+        itmax = std::max(itmax, it);
+        rmax = std::max(rmax, rr);
+    }
+    if (iters) *iters = itmax;
+    if (relres) *relres = rmax;
+    return SWE_OK;
+}
+
+int swe_implicit_euler_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps, int max_iter, double rel_tol,
+                             int *iters, double *relres)
+{
+    return implicit_steps(plan, v0, dt, nsteps, max_iter, rel_tol, iters, relres, false);
+}
+
+int swe_crank_nicolson_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps, int max_iter, double rel_tol,
+                             int *iters, double *relres)
+{
+    return implicit_steps(plan, v0, dt, nsteps, max_iter, rel_tol, iters, relres, true);
 }
 
 int swe_cfl_diag(swe_plan *plan, swe_state *st, double dt, double grav, double out[3])

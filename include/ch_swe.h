@@ -145,6 +145,35 @@ int swe_integrals_noalloc(swe_plan *plan, swe_state *st, double *mass, double *e
  * operator's).  Fused plans only; blocks until the values are on the host. */
 int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7]);
 
+/* operator_swm_lin_t%solve
+ * (models/shallow_water/operator/operator_swm_lin_mod.f90 L110-146): vout
+ * = (I - dt L0)^-1 rhs for the plan's linear operator without its Coriolis
+ * term (L0), as the reference solves it: the Helmholtz problem for h,
+ * (I - H g dt^2 div co2contra grad) h = rhs_h - H dt div co2contra(rhs_u,
+ * rhs_v), by conjugate gradients (solver/cg_solver_mod.f90 L19-90: initial
+ * guess the right-hand side, relative residual in the h quadrature's l2
+ * norm below rel_tol, at most max_iter iterations; the helm_solver_cfg
+ * keys rel_tol and max_iter), then u = rhs_u - g dt grad_x h, v = rhs_v -
+ * g dt grad_y h.  *iters (may be NULL): iterations done, 0 when the initial
+ * guess meets rel_tol, max_iter + 1 when the tolerance was never met (the
+ * result is then the last iterate, as in the reference, which only prints
+ * a message); *relres (may be NULL): the final relative residual.  vout and
+ * rhs must differ.  Device scratch (three states, four h fields) is
+ * allocated on the first call; blocks once per iteration for the
+ * convergence test. */
+int swe_solve(swe_plan *plan, swe_state *vout, swe_state *rhs, double dt, int max_iter, double rel_tol, int *iters,
+              double *relres);
+
+/* timescheme_t%step of implicit_Eul (time_schemes/implicit_Eul_mod.f90
+ * L20-30: v0 = solve(v0, dt)) and crank_nikolson (crank_nikolson_mod.f90
+ * L20-32: v0 = solve(v0 + dt/2 apply(v0), dt/2)), nsteps times, with the
+ * solver settings of swe_solve; *iters / *relres (may be NULL): the
+ * largest over the steps. */
+int swe_implicit_euler_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps, int max_iter, double rel_tol,
+                             int *iters, double *relres);
+int swe_crank_nicolson_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps, int max_iter, double rel_tol,
+                             int *iters, double *relres);
+
 /* swm_output_diag_t%print_CFL_diag
  * (models/shallow_water/output_and_diag/swm_output_diag_mod.f90 L110-154),
  * covariant winds: out[3] = the gravity-wave CFL sqrt(max g h / dx^2) dt

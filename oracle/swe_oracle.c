@@ -1191,6 +1191,83 @@ void swo_integrals_diag(swo *s, const double *u, const double *v, const double *
     out[6] = ens;
 }
 
+/* ---- the implicit solve (operator_swm_lin_mod.f90 solve L110-146) ---- */
+
+static double xy_dot(const swo *s, const double *a, const double *b)
+{
+    const int n = swo_npts(s, SWO_MESH_XY);
+    const double *w = s->G[SWO_MESH_XY].w;
+    double d = 0;
+    for (int k = 0; k < n; k++) d += w[k] * a[k] * b[k];
+    return d;
+}
+
+/* swm_helm_lin_operator_t%apply (swm_helm_operator_mod.f90 L56-81):
+ * out = z - gamma div(co2contra(grad z)) */
+static void helm_apply(swo *s, const double *z, double gamma, double *out, double *w)
+{
+    const int nu = swo_npts(s, SWO_MESH_U), nv = swo_npts(s, SWO_MESH_V), nh = swo_npts(s, SWO_MESH_XY);
+    double *gx = w, *gy = gx + nu, *gxt = gy + nv, *gyt = gxt + nu;
+    swo_grad(s, z, gx, gy);
+    swo_co2contra(s, gx, gy, gxt, gyt);
+    swo_div(s, gxt, gyt, out);
+    for (int k = 0; k < nh; k++) out[k] = (-1.0 * (g_mutation == 13 ? -gamma : gamma)) * out[k];
+    for (int k = 0; k < nh; k++) out[k] = out[k] + 1.0 * z[k];
+}
+
+/* operator_swm_lin_t%solve: vout = (I - dt L0)^-1 rhs for the linear
+ * operator without Coriolis (L0), through the Helmholtz problem for h,
+ * (I - H g dt^2 div co2contra grad) h = rhs_h - H dt div(co2contra(rhs_u,
+ * rhs_v)), solved by cg_solver_t%solve (cg_solver_mod.f90 L19-90: x0 = the
+ * right-hand side, relative residual in the h quadrature's l2 norm), then
+ * u = rhs_u - g dt grad_x h, v = rhs_v - g dt grad_y h.  *iters: the CG
+ * iterations done (0 when x0 already meets the tolerance, max_iter + 1 when
+ * it never does); *relres: the final relative residual. */
+void swo_solve(swo *s, const double *ru, const double *rv, const double *rh, double dt, double rel_tol, int max_iter,
+               double *u, double *v, double *h, int *iters, double *relres)
+{
+    const int nu = swo_npts(s, SWO_MESH_U), nv = swo_npts(s, SWO_MESH_V), nh = swo_npts(s, SWO_MESH_XY);
+    double *w = malloc((size_t)(2 * nu + 2 * nv) * sizeof(double));
+    double *b = malloc(nh * sizeof(double)), *r = malloc(nh * sizeof(double)), *p = malloc(nh * sizeof(double));
+    double *Ap = malloc(nh * sizeof(double)), *ut = malloc(nu * sizeof(double)), *vt = malloc(nv * sizeof(double));
+    swo_co2contra(s, ru, rv, ut, vt);
+    swo_div(s, ut, vt, b);
+    for (int k = 0; k < nh; k++) b[k] = 1.0 * rh[k] + (-s->H0 * dt) * b[k];
+    const double gamma = s->H0 * s->g * (dt * dt);
+    double *x = h;
+    memcpy(x, b, nh * sizeof(double));
+    helm_apply(s, x, gamma, r, w);
+    for (int k = 0; k < nh; k++) r[k] = -1.0 * r[k] + 1.0 * b[k];
+    double rn = sqrt(xy_dot(s, r, r));
+    const double bn = sqrt(xy_dot(s, b, b));
+    int it = 0;
+    /* a zero right-hand side has the solution zero (the reference's 0 / 0
+     * would go on with NaN) */
+    double rel = bn == 0.0 ? 0.0 : rn / bn;
+    if (!(rel < rel_tol)) {
+        memcpy(p, r, nh * sizeof(double));
+        for (it = 1; it <= max_iter; it++) {
+            helm_apply(s, p, gamma, Ap, w);
+            const double alpha = rn * rn / xy_dot(s, p, Ap);
+            for (int k = 0; k < nh; k++) x[k] = x[k] + alpha * p[k];
+            for (int k = 0; k < nh; k++) r[k] = r[k] + (-alpha) * Ap[k];
+            const double rnn = sqrt(xy_dot(s, r, r));
+            rel = rnn / bn;
+            if (rel < rel_tol) break;
+            const double beta = (rnn / rn) * (rnn / rn);
+            rn = rnn;
+            for (int k = 0; k < nh; k++) p[k] = 1.0 * r[k] + beta * p[k];
+        }
+    }
+    *iters = it;
+    *relres = rel;
+    double *gx = w, *gy = w + nu;
+    swo_grad(s, x, gx, gy);
+    for (int k = 0; k < nu; k++) u[k] = 1.0 * ru[k] + (-s->g * dt) * gx[k];
+    for (int k = 0; k < nv; k++) v[k] = 1.0 * rv[k] + (-s->g * dt) * gy[k];
+    free(w); free(b); free(r); free(p); free(Ap); free(ut); free(vt);
+}
+
 /* print_CFL_diag (swm_output_diag_mod.f90 L110-154), covariant winds: the
  * gravity-wave CFL sqrt(max(g / dx^2 h)) dt and the advective ones
  * sqrt(max |(ut / dx^2) u|) dt, sqrt(max |(vt / dy^2) v|) dt, ut and vt the

@@ -72,6 +72,10 @@ void swo_integrals_diag(swo *s, const double *u, const double *v, const double *
 /* h (packed xy layout) on the nlat x nlon lat-lon grid as the shallow-water
  * output writes h.dat: float, longitude fastest */
 void swo_latlon_h(swo *s, const double *h, int nlat, int nlon, float *out);
+/* the implicit solve of the linear operator without Coriolis (CG on the
+ * Helmholtz problem for h); h must not alias rh */
+void swo_solve(swo *s, const double *ru, const double *rv, const double *rh, double dt, double rel_tol, int max_iter,
+               double *u, double *v, double *h, int *iters, double *relres);
 /* print_CFL_diag: gravity-wave CFL, advective CFL along x and y */
 void swo_cfl_diag(swo *s, const double *u, const double *v, const double *h, double dt, double grav, double out[3]);
 /* print_integral_tends of the operator's tendency: out = TE_tend,

@@ -54,6 +54,8 @@ def _load():
     lib.swo_curl.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     lib.swo_integrals_diag.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
     lib.swo_integral_tends.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
+    lib.swo_solve.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                              _dp, _dp, _dp, _ip, _dp]
     lib.swo_cfl_diag.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
     lib.swo_latlon_h.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     lib.swo_gaussian_hill.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]
@@ -224,6 +226,17 @@ class Oracle:
         out = np.zeros(4)
         self.lib.swo_integral_tends(self._h, _p(u), _p(v), _p(h), grav, _p(out))
         return tuple(float(x) for x in (out if scales else out[:2]))
+
+    def solve(self, u, v, h, dt, rel_tol=1e-10, max_iter=200):
+        """operator_swm_lin_t%solve: (u, v, h) = (I - dt L0)^-1 rhs without
+        Coriolis, by CG on the Helmholtz problem; returns (u, v, h, iters,
+        relres)."""
+        u, v, h = self._c(u), self._c(v), self._c(h)
+        out = [np.empty_like(u), np.empty_like(v), np.empty_like(h)]
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        self.lib.swo_solve(self._h, _p(u), _p(v), _p(h), dt, rel_tol, max_iter, *[_p(a) for a in out],
+                           ctypes.byref(it), ctypes.byref(rr))
+        return out[0], out[1], out[2], it.value, rr.value
 
     def cfl_diag(self, u, v, h, dt, grav=9.80616):
         """print_CFL_diag: (gravity-wave CFL, advective CFL x, y)."""

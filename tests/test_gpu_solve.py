@@ -1,0 +1,139 @@
+"""GPU: the implicit solve (swe_solve, operator_swm_lin_mod.f90 solve
+L110-146 with cg_solver_mod.f90 L19-90) and the implicit time schemes built
+on it (implicit_Eul_mod.f90, crank_nikolson_mod.f90) against the oracle.
+Both sides run CG to a tight tolerance, so they agree to about the
+tolerance times the problem's conditioning, not bitwise (their sums round
+differently)."""
+import sys
+
+import numpy as np
+import pytest
+
+import ch_swe_amd as cs
+from oracle.swe_oracle import MESH_U, MESH_V, MESH_XY
+
+pytestmark = pytest.mark.gpu
+
+A = 6371229.0
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _rhs(o, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(o.shape(MESH_U)), rng.standard_normal(o.shape(MESH_V)),
+            rng.standard_normal(o.shape(MESH_XY)))
+
+
+def _smooth(o):
+    h0 = o.gaussian_hill(0.3, 0.4, 0.2, 10.0)
+    return o.rk4(o.zeros(MESH_U), o.zeros(MESH_V), h0, 600.0, 5)
+
+
+@pytest.mark.parametrize("order,N", [(21, 20), (42, 20), (42, 33), (21, 48)])
+def test_solve_matches_oracle(oracle_mod, order, N):
+    _say(f"solve order={order} N={N}")
+    o = oracle_mod.Oracle(N, order, radius=A, grav=9.80616, h_mean=1000.0)
+    op = cs.create_linear_swm_operator(N, order, cs.FP64, radius=A, grav=9.80616, h_mean=1000.0)
+    rhs = _rhs(o, 30 + N)
+    dt = 3000.0
+    ref = o.solve(*rhs, dt, 1e-12, 500)
+    r = op.create_stvec().upload(*rhs)
+    x = op.create_stvec()
+    it, rel = op.solve(x, r, dt, rel_tol=1e-12, max_iter=500)
+    assert rel < 1e-12 and abs(it - ref[3]) <= 2, (it, ref[3], rel)
+    for g, e in zip(x.download(), ref[:3]):
+        assert _rel(g, e) < 1e-9
+    # the right-hand side is left as it was
+    for g, e in zip(r.download(), rhs):
+        assert np.array_equal(g, e)
+
+
+@pytest.mark.parametrize("order", [21, 42])
+def test_solve_is_implicit_euler(order):
+    """Without rotation the result satisfies v - dt apply(v) = rhs: to
+    rounding in u, v and to the CG tolerance in h."""
+    _say(f"solve identity order={order}")
+    N, dt = 24, 2500.0
+    op = cs.create_linear_swm_operator(N, order, cs.FP64, radius=A, omega=0.0)
+    rng = np.random.default_rng(order)
+    shp = [(6, N + 1, N), (6, N, N + 1), (6, N + 1, N + 1)]
+    rhs = [rng.standard_normal(s) for s in shp]
+    r = op.create_stvec().upload(*rhs)
+    x, t = op.create_stvec(), op.create_stvec()
+    it, rel = op.solve(x, r, dt, rel_tol=1e-11, max_iter=500)
+    op.apply(t, x)
+    for xv, tv, rv, bar in zip(x.download(), t.download(), rhs, (1e-13, 1e-13, 3e-9)):
+        assert _rel(xv - dt * tv, rv) < bar
+
+
+@pytest.mark.parametrize("scheme", ["implicit_Eul", "crank_nikolson"])
+def test_implicit_schemes_match_oracle(oracle_mod, scheme):
+    """Three steps of each scheme against the oracle's composition of the
+    same apply / assign / solve calls; Crank-Nicolson without rotation keeps
+    the energy of a consistent state."""
+    _say(f"scheme {scheme}")
+    N, order, dt = 24, 42, 2000.0
+    o = oracle_mod.Oracle(N, order, radius=A, omega=0.0)
+    op = cs.create_linear_swm_operator(N, order, cs.FP64, radius=A, omega=0.0)
+    v0 = _smooth(o)
+    ref = [a.copy() for a in v0]
+    for _ in range(3):
+        if scheme == "implicit_Eul":
+            ref = list(o.solve(*ref, dt, 1e-12, 500)[:3])
+        else:
+            d = o.apply(*ref)
+            ref = list(o.solve(*[a + 0.5 * dt * b for a, b in zip(ref, d)], 0.5 * dt, 1e-12, 500)[:3])
+    st = op.create_stvec().upload(*v0)
+    ts = cs.create_timescheme(scheme, rel_tol=1e-12, max_iter=500)
+    ts.step(st, op, dt, 3)
+    assert 0 < ts.iters <= 500 and ts.relres < 1e-12
+    got = st.download()
+    for g, e in zip(got, ref):
+        assert _rel(g, e) < 1e-9
+    if scheme == "crank_nikolson":
+        assert abs(o.energy(*got) / o.energy(*v0) - 1) < 1e-11
+
+
+def test_solve_fp32_plan(oracle_mod):
+    """An FP32 plan's solve converges to an FP32-sized tolerance and agrees
+    with the FP64 oracle to that size."""
+    _say("solve fp32")
+    N, order, dt = 32, 42, 3000.0
+    o = oracle_mod.Oracle(N, order, radius=A)
+    op = cs.create_linear_swm_operator(N, order, cs.FP32, radius=A)
+    rhs = _rhs(o, 77)
+    ref = o.solve(*rhs, dt, 1e-12, 500)
+    x = op.create_stvec()
+    it, rel = op.solve(x, op.create_stvec().upload(*rhs), dt, rel_tol=1e-5, max_iter=500)
+    assert rel < 1e-5
+    for g, e in zip(x.download(), ref[:3]):
+        assert _rel(g, e) < 1e-4
+
+
+def test_solve_edge_cases():
+    """A zero right-hand side returns zero at once; a too small max_iter
+    reports max_iter + 1 iterations and a residual above the tolerance;
+    vout aliasing rhs is refused."""
+    _say("solve edge cases")
+    N = 12
+    op = cs.create_linear_swm_operator(N, 21, cs.FP64)
+    z = op.create_stvec().upload(np.zeros((6, N + 1, N)), np.zeros((6, N, N + 1)), np.zeros((6, N + 1, N + 1)))
+    x = op.create_stvec()
+    it, rel = op.solve(x, z, 1000.0)
+    assert it == 0 and rel == 0.0
+    assert not any(np.any(a) for a in x.download())
+    rng = np.random.default_rng(1)
+    r = op.create_stvec().upload(rng.standard_normal((6, N + 1, N)), rng.standard_normal((6, N, N + 1)),
+                                 rng.standard_normal((6, N + 1, N + 1)))
+    it, rel = op.solve(x, r, 5000.0, rel_tol=1e-14, max_iter=3)
+    assert it == 4 and rel >= 1e-14
+    with pytest.raises(cs.SweError):
+        op.solve(r, r, 1000.0)
