@@ -30,6 +30,7 @@ module swe_hip_capi_mod
               swe_state_upload, swe_state_download, swe_state_upload_halo, swe_state_download_halo, &
               swe_apply, swe_assign, swe_update,  &
               swe_rk4_steps, swe_integrals, swe_integrals_diag, swe_integral_tends, swe_latlon_h, swe_cfl_diag, &
+              swe_solve, swe_implicit_euler_steps, swe_crank_nicolson_steps, &
               swe_synchronize, swe_error_message
 
     interface
@@ -108,6 +109,33 @@ module swe_hip_capi_mod
             type(c_ptr), value :: plan, st
             real(c_double), value :: grav
             real(c_double), intent(out) :: out(7)
+        end function
+        integer(c_int) function swe_solve(plan, vout, rhs, dt, max_iter, rel_tol, iters, relres) &
+                bind(C, name="swe_solve")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: plan, vout, rhs
+            real(c_double), value :: dt, rel_tol
+            integer(c_int), value :: max_iter
+            integer(c_int), intent(out) :: iters
+            real(c_double), intent(out) :: relres
+        end function
+        integer(c_int) function swe_implicit_euler_steps(plan, v0, dt, nsteps, max_iter, rel_tol, iters, relres) &
+                bind(C, name="swe_implicit_euler_steps")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: plan, v0
+            real(c_double), value :: dt, rel_tol
+            integer(c_int), value :: nsteps, max_iter
+            integer(c_int), intent(out) :: iters
+            real(c_double), intent(out) :: relres
+        end function
+        integer(c_int) function swe_crank_nicolson_steps(plan, v0, dt, nsteps, max_iter, rel_tol, iters, relres) &
+                bind(C, name="swe_crank_nicolson_steps")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: plan, v0
+            real(c_double), value :: dt, rel_tol
+            integer(c_int), value :: nsteps, max_iter
+            integer(c_int), intent(out) :: iters
+            real(c_double), intent(out) :: relres
         end function
         integer(c_int) function swe_cfl_diag(plan, st, dt, grav, out) bind(C, name="swe_cfl_diag")
             import :: c_int, c_ptr, c_double

@@ -25,8 +25,8 @@ module swe_hip_mod
 
     public :: SWE_FP64, SWE_FP32
     public :: domain_hip_t, stvec_t, operator_t, timescheme_t
-    public :: stvec_hip_t, operator_swm_lin_hip_t, rk4_hip_t
-    public :: create_domain_hip, create_linear_swm_operator_hip, create_stvec_hip, create_rk4_hip
+    public :: stvec_hip_t, operator_swm_lin_hip_t, rk4_hip_t, implicit_hip_t
+    public :: create_domain_hip, create_linear_swm_operator_hip, create_stvec_hip, create_rk4_hip, create_implicit_hip
     public :: swe_hip_abort
 
     ! the "domain": geometry, edge maps and stream live in the device plan
@@ -109,6 +109,7 @@ module swe_hip_mod
         procedure :: integral_tends => operator_integral_tends
         procedure :: latlon_h => operator_latlon_h
         procedure :: cfl_diag => operator_cfl_diag
+        procedure :: solve_h => operator_solve
     end type operator_swm_lin_hip_t
 
     type, extends(timescheme_t) :: rk4_hip_t
@@ -116,6 +117,20 @@ module swe_hip_mod
         procedure :: step => rk4_step
         procedure :: steps => rk4_steps
     end type rk4_hip_t
+
+    ! implicit_Eul_t / crank_nikolson_t on the operator's solve, with the
+    ! helm_solver_cfg settings; iters / relres: the largest over the last
+    ! call's steps
+    type, extends(timescheme_t) :: implicit_hip_t
+        logical :: crank_nicolson = .false.
+        integer :: max_iter = 100
+        real(kind=8) :: rel_tol = 1.0e-10_8
+        integer :: iters = 0
+        real(kind=8) :: relres = 0.0_8
+    contains
+        procedure :: step => implicit_step
+        procedure :: steps => implicit_steps
+    end type implicit_hip_t
 
 contains
 
@@ -178,6 +193,17 @@ contains
     function create_rk4_hip() result(ts)
         type(rk4_hip_t) :: ts
     end function create_rk4_hip
+
+    ! timescheme_factory_mod.f90 L31-34: 'implicit_Eul' or 'crank_nikolson'
+    function create_implicit_hip(crank_nicolson, max_iter, rel_tol) result(ts)
+        logical,      intent(in) :: crank_nicolson
+        integer,      intent(in) :: max_iter
+        real(kind=8), intent(in) :: rel_tol
+        type(implicit_hip_t) :: ts
+        ts%crank_nicolson = crank_nicolson
+        ts%max_iter = max_iter
+        ts%rel_tol = rel_tol
+    end function create_implicit_hip
 
     subroutine create_stvec_hip(v, domain)
         type(stvec_hip_t),  intent(out) :: v
@@ -316,6 +342,52 @@ contains
         real(kind=8),                  intent(out)   :: cfl(3)
         call check(swe_cfl_diag(domain%plan, hip_state(v), dt, grav, cfl), "cfl_diag")
     end subroutine operator_cfl_diag
+
+    ! operator_swm_lin_mod.f90 solve L110-146: vout = (I - dt L0)^-1 rhs (no
+    ! Coriolis), CG on the Helmholtz problem for h
+    subroutine operator_solve(this, vout, rhs, domain, dt, max_iter, rel_tol, iters, relres)
+        class(operator_swm_lin_hip_t), intent(inout) :: this
+        class(stvec_t),                intent(inout) :: vout
+        class(stvec_t),                intent(in)    :: rhs
+        type(domain_hip_t),            intent(in)    :: domain
+        real(kind=8),                  intent(in)    :: dt, rel_tol
+        integer,                       intent(in)    :: max_iter
+        integer,                       intent(out)   :: iters
+        real(kind=8),                  intent(out)   :: relres
+        integer(c_int) :: it
+        call check(swe_solve(domain%plan, hip_state(vout), hip_state(rhs), dt, int(max_iter, c_int), rel_tol, it, &
+                             relres), "solve")
+        iters = it
+    end subroutine operator_solve
+
+    subroutine implicit_step(this, v0, operator, domain, dt)
+        class(implicit_hip_t), intent(inout) :: this
+        class(stvec_t),        intent(inout) :: v0
+        class(operator_t),     intent(inout) :: operator
+        type(domain_hip_t),    intent(in)    :: domain
+        real(kind=8),          intent(in)    :: dt
+        call this%steps(v0, domain, dt, 1)
+    end subroutine implicit_step
+
+    ! implicit_Eul_mod.f90 L20-30 / crank_nikolson_mod.f90 L20-32, nsteps times
+    subroutine implicit_steps(this, v0, domain, dt, nsteps)
+        class(implicit_hip_t), intent(inout) :: this
+        class(stvec_t),        intent(inout) :: v0
+        type(domain_hip_t),    intent(in)    :: domain
+        real(kind=8),          intent(in)    :: dt
+        integer,               intent(in)    :: nsteps
+        integer(c_int) :: it
+        if (this%crank_nicolson) then
+            call check(swe_crank_nicolson_steps(domain%plan, hip_state(v0), dt, int(nsteps, c_int), &
+                                                int(this%max_iter, c_int), this%rel_tol, it, this%relres), &
+                       "crank_nikolson steps")
+        else
+            call check(swe_implicit_euler_steps(domain%plan, hip_state(v0), dt, int(nsteps, c_int), &
+                                                int(this%max_iter, c_int), this%rel_tol, it, this%relres), &
+                       "implicit_Eul steps")
+        end if
+        this%iters = it
+    end subroutine implicit_steps
 
     ! rk4_mod.f90 step_rk4 L20-42 (fused stages on the device)
     subroutine rk4_step(this, v0, operator, domain, dt)

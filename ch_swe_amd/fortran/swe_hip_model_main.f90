@@ -17,7 +17,10 @@
 !            co2contra_op_name 'co2contra_ch_sbpXX', coriolis_op_name
 !            'coriolis_Ch_sbpXX', quadrature_name 'SBP_CXX_quadrature' with
 !            one XX in {21, 42} (operator_swm_factory_mod.f90 L586-685);
-!   timescheme_name 'rk4';
+!   timescheme_name 'rk4', 'implicit_Eul' or 'crank_nikolson' (the last two
+!            with the operator section's helm_solver_cfg: solver_name 'cg',
+!            max_iter, rel_tol, verbose; iterative_solver_factory_mod.f90
+!            L27-36);
 !   testcase (testcase_section -> testcase_name): 'Gaussian_hill_linear'
 !            (phi0, lam0, d0, refsol_path; gaussian_hill_linear_test_mod.f90
 !            L30-67) or 'Williamson_test2' / 'TS2' / 'test2' with linear_mode
@@ -62,6 +65,8 @@ program swe_hip_model
     type(domain_hip_t)           :: domain
     type(operator_swm_lin_hip_t) :: operator
     type(rk4_hip_t)              :: timescheme
+    type(implicit_hip_t)         :: implicit_ts
+    logical                      :: implicit, cg_verbose
     type(stvec_hip_t)            :: state
     real(kind=8), allocatable    :: u(:,:,:), v(:,:,:), h(:,:,:), h0(:,:,:), jac(:,:)
     character(len=:), allocatable :: opsec, tcsec, tcname, name, refsol_path
@@ -147,7 +152,16 @@ program swe_hip_model
     call same_order(sbp_order(opsec//"%coriolis_op_name", "coriolis_Ch_sbp", ""), opsec//"%coriolis_op_name")
     call same_order(sbp_order(opsec//"%quadrature_name", "SBP_C", "_quadrature"), opsec//"%quadrature_name")
 
-    call expect_str("timescheme_name", "", "rk4")
+    name = config%get_str("timescheme_name")
+    implicit = name == "implicit_Eul" .or. name == "crank_nikolson"
+    if (.not. implicit .and. name /= "rk4") &
+        call swe_hip_abort("timescheme_name "//name//": only rk4, implicit_Eul and crank_nikolson run here")
+    if (implicit) then
+        call expect_str(opsec//"%helm_solver_cfg%solver_name", "", "cg")
+        implicit_ts = create_implicit_hip(name == "crank_nikolson", config%get_int(opsec//"%helm_solver_cfg%max_iter"), &
+                                          config%get_real(opsec//"%helm_solver_cfg%rel_tol"))
+        cg_verbose = config%get_logical(opsec//"%helm_solver_cfg%verbose", .false.)
+    end if
     name = config%get_str("diffusion_section", "None")
     if (name /= "None") call swe_hip_abort("diffusion_section: horizontal diffusion is not supported")
 
@@ -254,7 +268,18 @@ program swe_hip_model
     do while (it < nsteps)
         ! run up to the next diagnostics / write step in one call
         nrun = min(nsteps, (it / nstep_diagnostics + 1) * nstep_diagnostics, (it / nstep_write + 1) * nstep_write) - it
-        call timescheme%steps(state, domain, dt, nrun)
+        if (implicit) then
+            call implicit_ts%steps(state, domain, dt, nrun)
+            ! cg_solver_mod.f90 L82-88, once per chunk of steps (the largest
+            ! iteration count over them)
+            if (implicit_ts%iters > implicit_ts%max_iter) then
+                print *, "CG failed to converge in ", implicit_ts%max_iter, " iterations"
+            else if (cg_verbose) then
+                print *, "CG converged in ", implicit_ts%iters, " iterations"
+            end if
+        else
+            call timescheme%steps(state, domain, dt, nrun)
+        end if
         it = it + nrun
         if (mod(it, nstep_diagnostics) == 0) then
             ! print_integrals_diag (swm_output_diag_mod.f90 L156-184) and

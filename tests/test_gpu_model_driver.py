@@ -148,3 +148,30 @@ def test_long_chunk_runs_through_graph_replay(oracle_mod, tmp_path):
     ref = o.rk4(o.zeros(MESH_U), o.zeros(MESH_V), h0, 900.0, 80)
     for g, r in zip(_split(raw, N), ref):
         assert _rel(g, r) < 1e-11
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["implicit_Eul", "crank_nikolson"])
+def test_implicit_schemes_in_the_model_program(oracle_mod, tmp_path, scheme):
+    """timescheme_name 'implicit_Eul' / 'crank_nikolson' with the operator
+    section's helm_solver_cfg (cg, tightened to rel_tol 1e-12): the final
+    state against the oracle's composition of the same apply / solve calls
+    over 8 steps; verbose prints the CG iteration lines."""
+    text = GAUSS_CFG.replace("timescheme_name = 'rk4'", f"timescheme_name = '{scheme}'") \
+        .replace("max_iter = 100,\n            rel_tol  = 0.00001",
+                 "max_iter = 500,\n            rel_tol  = 1e-12,\n            verbose = .true.")
+    stdout, raw = _run(tmp_path, text, "fp64")
+    N, dt = 32, 900.0
+    h_mean = (2 * math.pi * EARTH_RADII / (5 * 86400.0)) ** 2 / EARTH_GRAV
+    o = oracle_mod.Oracle(N, 21, radius=EARTH_RADII, grav=EARTH_GRAV, h_mean=h_mean, omega=1e-4,
+                          coriolis_const=True)
+    st = [o.zeros(MESH_U), o.zeros(MESH_V), o.gaussian_hill(0.25 * math.pi, math.asin(1 / math.sqrt(3)), 0.25, 1.0)]
+    for _ in range(8):
+        if scheme == "implicit_Eul":
+            st = list(o.solve(*st, dt, 1e-12, 500)[:3])
+        else:
+            d = o.apply(*st)
+            st = list(o.solve(*[a + 0.5 * dt * b for a, b in zip(st, d)], 0.5 * dt, 1e-12, 500)[:3])
+    for g, r in zip(_split(raw, N), st):
+        assert _rel(g, r) < 1e-9
+    assert len(re.findall(r"CG converged in\s+\d+\s+iterations", stdout)) == 2
