@@ -38,7 +38,7 @@ EXPORTS = (
     "swe_last_error", "swe_version", "swe_plan_create", "swe_plan_destroy", "swe_plan_npts",
     "swe_plan_stream", "swe_synchronize", "swe_state_create", "swe_state_destroy",
     "swe_state_upload", "swe_state_download", "swe_apply", "swe_assign", "swe_update",
-    "swe_rk4_steps", "swe_integrals", "swe_integrals_noalloc", "swe_integrals_diag", "swe_integral_tends", "swe_cfl_diag", "swe_latlon_h", "swe_solve", "swe_implicit_euler_steps",
+    "swe_rk4_steps", "swe_integrals", "swe_integrals_noalloc", "swe_integrals_diag", "swe_integral_tends", "swe_cfl_diag", "swe_latlon_h", "swe_latlon_uv", "swe_solve", "swe_implicit_euler_steps",
     "swe_crank_nicolson_steps", "swe_state_upload_halo", "swe_state_download_halo", "swe_halo_map", "swe_ensemble_create",
     "swe_ensemble_destroy", "swe_ensemble_rk4", "swe_ensemble_stream", "swe_ensemble_synchronize",
     "swe_plan_timing", "swe_plan_kernel_times", "swe_server_create", "swe_server_destroy",
@@ -110,6 +110,8 @@ def lib():
     L.swe_integral_tends.argtypes = [_vp, _vp, ctypes.c_double, _dp]
     L.swe_latlon_h.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     L.swe_cfl_diag.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_double, _dp]
+    L.swe_latlon_uv.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                ctypes.POINTER(ctypes.c_float)]
     L.swe_solve.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, _ip, _dp]
     for name in ("swe_implicit_euler_steps", "swe_crank_nicolson_steps"):
         getattr(L, name).argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, _dp]
@@ -345,6 +347,18 @@ class OperatorSwmLin:
         out = np.empty((nlat, nlon), np.float32)
         _check(lib().swe_latlon_h(self.plan.h, state.h, nlat, nlon, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
+
+    def latlon_uv(self, state, nlat=None, nlon=None):
+        """The zonal and meridional winds on the lat-lon grid as write_fields
+        writes u.dat / v.dat (swm_output_diag_mod.f90 L75-76,
+        latlon_outputer_mod.f90 L65-107): two float32 (nlat, nlon) arrays."""
+        N = self.cfg.N
+        nlat = 2 * N + 1 if nlat is None else int(nlat)
+        nlon = 4 * N if nlon is None else int(nlon)
+        uo, vo = np.empty((nlat, nlon), np.float32), np.empty((nlat, nlon), np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        _check(lib().swe_latlon_uv(self.plan.h, state.h, nlat, nlon, uo.ctypes.data_as(fp), vo.ctypes.data_as(fp)))
+        return uo, vo
 
     def integral_tends(self, state, grav=9.80616):
         """print_integral_tends (swm_output_diag_mod.f90 L236-342) of this

@@ -30,9 +30,17 @@ struct LatlonTables {
     // (x_ind-1, y_ind-1) and the fractional positions zdx - x_ind, zdy - y_ind
     std::vector<int> base;
     std::vector<double> fx, fy;
+    // winds only: per point (q11, q12, q21, q22), zonal = q11 u + q12 v,
+    // meridional = q21 u + q22 v of the interpolated covariant (u, v)
+    std::vector<double> q;
 };
 
-// returns 0, or 1 with a message (sizes out of range)
+// h: returns 0, or 1 with a message (sizes out of range)
 int build_latlon_tables(int N, int nlat, int nlon, LatlonTables &t, std::string &err);
+
+// the winds (u.dat, v.dat): src indexes the colocated covariant components
+// at the xy points, U then V (6 (N+1)^2 each, packed); the rows hold the U
+// component's extended arrays, then the V component's
+int build_latlon_vec_tables(int N, int nlat, int nlon, LatlonTables &t, std::string &err);
 
 }  // namespace swe

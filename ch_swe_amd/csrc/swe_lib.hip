@@ -452,6 +452,8 @@ struct EngineBase {
     virtual int integral_tends(void *st, double grav, double out[2]) = 0;
     // h on the nlat x nlon lat-lon grid as the output writes h.dat (host out)
     virtual int latlon_h(void *st, int nlat, int nlon, float *out) = 0;
+    // the zonal and meridional winds on it as the output writes u.dat, v.dat
+    virtual int latlon_uv(void *st, int nlat, int nlon, float *uo, float *vo) = 0;
     virtual int rk4(void *v0, void *acc, void *ya, void *yb, double dt, int nsteps) = 0;
     virtual bool fused() const = 0;
     virtual size_t elem() const = 0;
@@ -888,65 +890,94 @@ struct Engine : EngineBase {
         return SWE_OK;
     }
 
-    // lat-lon output: the tables of the last grid asked for, the extended
-    // arrays and the output (built on the first call for a grid)
-    struct {
+    // lat-lon output: per kind (0 h, 1 winds) the tables of the last grid
+    // asked for, the extended arrays and the outputs (built on the first
+    // call for a grid)
+    struct LL {
         int nlat = 0, nlon = 0, nrows = 0, np = 0, ew = 0;
         size_t nbytes = 0;
         int *rp = nullptr, *col = nullptr, *base = nullptr;
-        double *w = nullptr, *fx = nullptr, *fy = nullptr, *ext = nullptr;
+        double *w = nullptr, *fx = nullptr, *fy = nullptr, *q = nullptr, *ext = nullptr, *uv = nullptr;
         float *out = nullptr;
-    } ll;
-    int latlon_tables(int nlat, int nlon)
+    } ll[2];
+    int latlon_tables(int kind, int nlat, int nlon)
     {
-        if (ll.nlat == nlat && ll.nlon == nlon) return SWE_OK;
+        LL &L = ll[kind];
+        if (L.nlat == nlat && L.nlon == nlon) return SWE_OK;
         swe::LatlonTables t;
         std::string err;
-        if (swe::build_latlon_tables(d.N, nlat, nlon, t, err)) return fail(SWE_EINVAL, "swe_latlon_h: " + err);
+        if (kind == 0 ? swe::build_latlon_tables(d.N, nlat, nlon, t, err)
+                      : swe::build_latlon_vec_tables(d.N, nlat, nlon, t, err))
+            return fail(SWE_EINVAL, std::string(kind == 0 ? "swe_latlon_h: " : "swe_latlon_uv: ") + err);
         HIP_TRY(hipStreamSynchronize(pl->stream));
-        void *old[] = {ll.rp, ll.col, ll.base, ll.w, ll.fx, ll.fy, ll.ext, ll.out};
+        void *old[] = {L.rp, L.col, L.base, L.w, L.fx, L.fy, L.q, L.ext, L.uv, L.out};
         for (void *q : old)
             if (q) release(q);
-        bytes -= ll.nbytes;
-        ll = {};
+        bytes -= L.nbytes;
+        L = LL();
+        // h: sources are the state's h points (element offsets); winds:
+        // the colocated components' packed indices as they are
         const int n = d.N + 1;
         std::vector<int> col(t.src.size());
         for (size_t k = 0; k < col.size(); k++) {
             const int s = t.src[k], p = s / (n * n), r = s - p * n * n, j = r / n + 1, i = r - (j - 1) * n + 1;
-            col[k] = (int)off(d, p, i, j);
+            col[k] = kind == 0 ? (int)off(d, p, i, j) : s;
         }
-        const size_t nr = t.rp.size() - 1, np = t.base.size(), nnz = col.size();
+        const size_t nr = t.rp.size() - 1, np = t.base.size(), nnz = col.size(), nout = kind == 0 ? np : 2 * np;
         const size_t before = bytes;
         int rc;
-        if ((rc = alloc((void **)&ll.rp, (nr + 1) * sizeof(int))) || (rc = alloc((void **)&ll.col, nnz * sizeof(int))) ||
-            (rc = alloc((void **)&ll.w, nnz * sizeof(double))) || (rc = alloc((void **)&ll.base, np * sizeof(int))) ||
-            (rc = alloc((void **)&ll.fx, np * sizeof(double))) || (rc = alloc((void **)&ll.fy, np * sizeof(double))) ||
-            (rc = alloc((void **)&ll.ext, nr * sizeof(double))) || (rc = alloc((void **)&ll.out, np * sizeof(float))))
+        if ((rc = alloc((void **)&L.rp, (nr + 1) * sizeof(int))) || (rc = alloc((void **)&L.col, nnz * sizeof(int))) ||
+            (rc = alloc((void **)&L.w, nnz * sizeof(double))) || (rc = alloc((void **)&L.base, np * sizeof(int))) ||
+            (rc = alloc((void **)&L.fx, np * sizeof(double))) || (rc = alloc((void **)&L.fy, np * sizeof(double))) ||
+            (rc = alloc((void **)&L.ext, nr * sizeof(double))) || (rc = alloc((void **)&L.out, nout * sizeof(float))))
             return rc;
-        HIP_TRY(hipMemcpyAsync(ll.rp, t.rp.data(), (nr + 1) * sizeof(int), hipMemcpyHostToDevice, pl->stream));
-        HIP_TRY(hipMemcpyAsync(ll.col, col.data(), nnz * sizeof(int), hipMemcpyHostToDevice, pl->stream));
-        HIP_TRY(hipMemcpyAsync(ll.w, t.w.data(), nnz * sizeof(double), hipMemcpyHostToDevice, pl->stream));
-        HIP_TRY(hipMemcpyAsync(ll.base, t.base.data(), np * sizeof(int), hipMemcpyHostToDevice, pl->stream));
-        HIP_TRY(hipMemcpyAsync(ll.fx, t.fx.data(), np * sizeof(double), hipMemcpyHostToDevice, pl->stream));
-        HIP_TRY(hipMemcpyAsync(ll.fy, t.fy.data(), np * sizeof(double), hipMemcpyHostToDevice, pl->stream));
+        if (kind == 1 && ((rc = alloc((void **)&L.q, 4 * np * sizeof(double))) ||
+                          (rc = alloc((void **)&L.uv, 2 * (size_t)6 * n * n * sizeof(double)))))
+            return rc;
+        HIP_TRY(hipMemcpyAsync(L.rp, t.rp.data(), (nr + 1) * sizeof(int), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(L.col, col.data(), nnz * sizeof(int), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(L.w, t.w.data(), nnz * sizeof(double), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(L.base, t.base.data(), np * sizeof(int), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(L.fx, t.fx.data(), np * sizeof(double), hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipMemcpyAsync(L.fy, t.fy.data(), np * sizeof(double), hipMemcpyHostToDevice, pl->stream));
+        if (kind == 1)
+            HIP_TRY(hipMemcpyAsync(L.q, t.q.data(), 4 * np * sizeof(double), hipMemcpyHostToDevice, pl->stream));
         HIP_TRY(hipStreamSynchronize(pl->stream));
-        ll.nlat = nlat;
-        ll.nlon = nlon;
-        ll.nrows = (int)nr;
-        ll.np = (int)np;
-        ll.ew = t.ew;
-        ll.nbytes = bytes - before;
+        L.nlat = nlat;
+        L.nlon = nlon;
+        L.nrows = (int)nr;
+        L.np = (int)np;
+        L.ew = t.ew;
+        L.nbytes = bytes - before;
         return SWE_OK;
     }
     int latlon_h(void *st, int nlat, int nlon, float *out) override
     {
-        int rc = latlon_tables(nlat, nlon);
+        int rc = latlon_tables(0, nlat, nlon);
         if (rc) return rc;
+        LL &L = ll[0];
         const T *h = (const T *)st + 2 * d.F;
-        k_ll_ext<T><<<(ll.nrows + 255) / 256, 256, 0, pl->stream>>>(h, ll.rp, ll.col, ll.w, ll.nrows, ll.ext);
-        k_ll_interp<<<(ll.np + 255) / 256, 256, 0, pl->stream>>>(ll.ext, ll.base, ll.fx, ll.fy, ll.np, ll.ew, ll.out);
+        k_ll_ext<T><<<(L.nrows + 255) / 256, 256, 0, pl->stream>>>(h, L.rp, L.col, L.w, L.nrows, L.ext);
+        k_ll_interp<<<(L.np + 255) / 256, 256, 0, pl->stream>>>(L.ext, L.base, L.fx, L.fy, L.np, L.ew, L.out);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(out, ll.out, (size_t)ll.np * sizeof(float), hipMemcpyDeviceToHost, pl->stream));
+        HIP_TRY(hipMemcpyAsync(out, L.out, (size_t)L.np * sizeof(float), hipMemcpyDeviceToHost, pl->stream));
+        HIP_TRY(hipStreamSynchronize(pl->stream));
+        return SWE_OK;
+    }
+    int latlon_uv(void *st, int nlat, int nlon, float *uo, float *vo) override
+    {
+        int rc = latlon_tables(1, nlat, nlon);
+        if (rc) return rc;
+        LL &L = ll[1];
+        const T *u = (const T *)st, *v = u + d.F;
+        const long nxy = 6L * (d.N + 1) * (d.N + 1);
+        k_ll_colocate<T><<<(unsigned)((nxy + 255) / 256), 256, 0, pl->stream>>>(d, u, v, L.uv);
+        k_ll_ext<double><<<(L.nrows + 255) / 256, 256, 0, pl->stream>>>(L.uv, L.rp, L.col, L.w, L.nrows, L.ext);
+        k_ll_interp_uv<<<(L.np + 255) / 256, 256, 0, pl->stream>>>(L.ext, L.nrows / 2, L.base, L.fx, L.fy, L.q, L.np,
+                                                                  L.ew, L.out, L.out + L.np);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(uo, L.out, (size_t)L.np * sizeof(float), hipMemcpyDeviceToHost, pl->stream));
+        HIP_TRY(hipMemcpyAsync(vo, L.out + L.np, (size_t)L.np * sizeof(float), hipMemcpyDeviceToHost, pl->stream));
         HIP_TRY(hipStreamSynchronize(pl->stream));
         return SWE_OK;
     }
@@ -1533,6 +1564,14 @@ int swe_cfl_diag(swe_plan *plan, swe_state *st, double dt, double grav, double o
     int rc = check_state(plan, st, "state");
     if (rc) return rc;
     return plan->eng->cfl_diag(st->buf, dt, grav, out);
+}
+
+int swe_latlon_uv(swe_plan *plan, swe_state *st, int nlat, int nlon, float *u_out, float *v_out)
+{
+    if (!plan || !u_out || !v_out) return fail(SWE_EINVAL, "swe_latlon_uv: NULL argument");
+    int rc = check_state(plan, st, "state");
+    if (rc) return rc;
+    return plan->eng->latlon_uv(st->buf, nlat, nlon, u_out, v_out);
 }
 
 int swe_latlon_h(swe_plan *plan, swe_state *st, int nlat, int nlon, float *out)

@@ -29,7 +29,7 @@ module swe_hip_capi_mod
     public :: swe_plan_create, swe_plan_destroy, swe_state_create, swe_state_destroy, &
               swe_state_upload, swe_state_download, swe_state_upload_halo, swe_state_download_halo, &
               swe_apply, swe_assign, swe_update,  &
-              swe_rk4_steps, swe_integrals, swe_integrals_diag, swe_integral_tends, swe_latlon_h, swe_cfl_diag, &
+              swe_rk4_steps, swe_integrals, swe_integrals_diag, swe_integral_tends, swe_latlon_h, swe_latlon_uv, swe_cfl_diag, &
               swe_solve, swe_implicit_euler_steps, swe_crank_nicolson_steps, &
               swe_synchronize, swe_error_message
 
@@ -142,6 +142,12 @@ module swe_hip_capi_mod
             type(c_ptr), value :: plan, st
             real(c_double), value :: dt, grav
             real(c_double), intent(out) :: out(3)
+        end function
+        integer(c_int) function swe_latlon_uv(plan, st, nlat, nlon, uo, vo) bind(C, name="swe_latlon_uv")
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: plan, st
+            integer(c_int), value :: nlat, nlon
+            real(c_float), intent(out) :: uo(*), vo(*)
         end function
         integer(c_int) function swe_latlon_h(plan, st, nlat, nlon, out) bind(C, name="swe_latlon_h")
             import :: c_int, c_ptr, c_float

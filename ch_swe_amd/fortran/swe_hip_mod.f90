@@ -108,6 +108,7 @@ module swe_hip_mod
         procedure :: integrals_diag => operator_integrals_diag
         procedure :: integral_tends => operator_integral_tends
         procedure :: latlon_h => operator_latlon_h
+        procedure :: latlon_uv => operator_latlon_uv
         procedure :: cfl_diag => operator_cfl_diag
         procedure :: solve_h => operator_solve
     end type operator_swm_lin_hip_t
@@ -331,6 +332,18 @@ contains
         call check(swe_latlon_h(domain%plan, hip_state(v), int(size(hll, 2), c_int), int(size(hll, 1), c_int), hll), &
                    "latlon_h")
     end subroutine operator_latlon_h
+
+    ! swm_output_diag_mod.f90 write_fields L75-76 through latlon_outputer_mod.f90
+    ! latlon_vec_write L65-107: zonal ull and meridional vll winds, (nlon, nlat)
+    ! each, one record of u.dat / v.dat
+    subroutine operator_latlon_uv(this, v, domain, ull, vll)
+        class(operator_swm_lin_hip_t), intent(inout) :: this
+        class(stvec_t),                intent(in)    :: v
+        type(domain_hip_t),            intent(in)    :: domain
+        real(kind=4),                  intent(out)   :: ull(:, :), vll(:, :)
+        call check(swe_latlon_uv(domain%plan, hip_state(v), int(size(ull, 2), c_int), int(size(ull, 1), c_int), &
+                                 ull, vll), "latlon_uv")
+    end subroutine operator_latlon_uv
 
     ! swm_output_diag_mod.f90 print_CFL_diag L110-154: cfl = gravity-wave
     ! CFL, advective CFL along x and y

@@ -38,10 +38,10 @@
 ! TE, KE, PE, enstrophy, computed on the device (swe_integrals_diag) with the
 ! Ch operators of the paper's output_diag section, followed by the line of
 ! print_integral_tends (L236-342, "dynamics: TE_tend = ..., Enstrophy_tend =
-! ...", swe_integral_tends).  h.dat in the working directory gets the lat-lon
-! h (2N+1 latitudes, 4N longitudes, float32; swe_latlon_h) at t = 0 and at
-! every write step, as write_fields writes it (u.dat, v.dat are not
-! produced).
+! ...", swe_integral_tends).  h.dat, u.dat and v.dat in the working directory
+! get the lat-lon h and zonal / meridional winds (2N+1 latitudes, 4N
+! longitudes, float32; swe_latlon_h, swe_latlon_uv) at t = 0 and at every
+! write step, as write_fields writes them.
 ! The final state is written to out.bin (default swe_hip_state.bin) in the
 ! packed (u, v, h) layout of include/ch_swe.h.
 program swe_hip_model
@@ -75,7 +75,7 @@ program swe_hip_model
     real(kind=8) :: dt, tau_write, tau_diagnostics, simulation_time, radius, omega, h_mean, axis(3), rm(9)
     real(kind=8) :: lam0, phi0, d0, u0, tc_h_mean, an(3), c(3), r(3), al, be, X, Y, s, vel(3), a1(3), a2(3)
     real(kind=8) :: diag(7), tends(2), cfl(3), l2_h0, linf_h0
-    real(kind=4), allocatable :: hll(:, :)
+    real(kind=4), allocatable :: hll(:, :), ull(:, :), vll(:, :)
     logical :: coriolis_const, linear_mode, gauss, have_refsol
 
     call get_command_argument(1, cfg_file)
@@ -261,7 +261,7 @@ program swe_hip_model
     ! state after k write intervals
     call operator%cfl_diag(state, domain, dt, Earth_grav, cfl)
     print '(A,F15.7,A,2F15.7)', "Gravity waves CFL-max = ", cfl(1), ", Advective CFL-max (x,y) = ", cfl(2), cfl(3)
-    allocate(hll(4 * N, 2 * N + 1))
+    allocate(hll(4 * N, 2 * N + 1), ull(4 * N, 2 * N + 1), vll(4 * N, 2 * N + 1))
     call write_h(1)
     call print_scores(0)
     it = 0
@@ -415,21 +415,33 @@ contains
         write(*, '(A)') trim(str_buff)
     end subroutine print_scores
 
-    ! latlon_outputer_mod.f90 latlon_write L33-63: record irec of the
-    ! direct-access float32 file h.dat (record length nlat * nlon values,
-    ! longitude fastest); the file is started afresh with record 1
+    ! write_fields (swm_output_diag_mod.f90 L52-76) through
+    ! latlon_outputer_mod.f90 latlon_write / latlon_vec_write L33-107: record
+    ! irec of the direct-access float32 files h.dat, u.dat, v.dat (record
+    ! length nlat * nlon values, longitude fastest); the files are started
+    ! afresh with record 1
     subroutine write_h(irec)
         integer, intent(in) :: irec
-        integer :: unit, rl
         call operator%latlon_h(state, domain, hll)
-        inquire(iolength=rl) hll
-        if (irec == 1) then
-            open(newunit=unit, file="h.dat", access="direct", form="unformatted", recl=rl, status="replace")
-        else
-            open(newunit=unit, file="h.dat", access="direct", form="unformatted", recl=rl, status="old")
-        end if
-        write(unit, rec=irec) hll
-        close(unit)
+        call operator%latlon_uv(state, domain, ull, vll)
+        call write_record("h.dat", hll, irec)
+        call write_record("u.dat", ull, irec)
+        call write_record("v.dat", vll, irec)
     end subroutine write_h
+
+    subroutine write_record(file, f, irec)
+        character(len=*), intent(in) :: file
+        real(kind=4),     intent(in) :: f(:, :)
+        integer,          intent(in) :: irec
+        integer :: unit, rl
+        inquire(iolength=rl) f
+        if (irec == 1) then
+            open(newunit=unit, file=file, access="direct", form="unformatted", recl=rl, status="replace")
+        else
+            open(newunit=unit, file=file, access="direct", form="unformatted", recl=rl, status="old")
+        end if
+        write(unit, rec=irec) f
+        close(unit)
+    end subroutine write_record
 
 end program swe_hip_model

@@ -196,6 +196,18 @@ int swe_cfl_diag(swe_plan *plan, swe_state *st, double dt, double grav, double o
  * point); blocks until out is filled. */
 int swe_latlon_h(swe_plan *plan, swe_state *st, int nlat, int nlon, float *out);
 
+/* The lat-lon output of the winds: write_fields's u.dat / v.dat
+ * (swm_output_diag_mod.f90 L75-76) through latlon_vec_outputer_t%write
+ * (latlon_outputer_mod.f90 L65-107) and do_latlon_vector_regrid
+ * (regridders/latlon_regrid_mod.f90 L166-262): the covariant u, v colocated
+ * at the xy points (interp2d_uv2pvec_Ch_sbp42), the vector ECS halo
+ * (equiang_cs/ecs_halo_vec_xy_mod.f90, covariant), cubic interpolation and
+ * the turn into zonal (u_out) and meridional (v_out) components, each nlat *
+ * nlon floats laid out as swe_latlon_h's.  Tables are built on the first
+ * call for a grid (about 150 bytes per lat-lon point); blocks until both
+ * outputs are filled. */
+int swe_latlon_uv(swe_plan *plan, swe_state *st, int nlat, int nlon, float *u_out, float *v_out);
+
 /* swm_output_diag_t%print_integral_tends
  * (models/shallow_water/output_and_diag/swm_output_diag_mod.f90 L236-342) of
  * the plan's operator: the tendency apply(st) goes to a plan-owned buffer

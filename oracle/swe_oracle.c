@@ -1508,6 +1508,147 @@ void swo_latlon_h(swo *s, const double *h, int nlat, int nlon, float *out)
     free(m.e);
 }
 
+/* init_transform_matrix (ecs_halo_vec_xy_factory_mod.f90 L115-187),
+ * covariant components: at the exchanged point of edge e (0 bottom, 1 top,
+ * 2 left, 3 right), depth j (1: the neighbour's copy of the edge point),
+ * along index i, the 2 x 2 map from the neighbour's (frame-aligned)
+ * components to the target panel's extended-coordinate components:
+ * tm = [b1s.a1t, b2s.a1t; b1s.a2t, b2s.a2t], b the neighbour's contravariant
+ * basis rotated into the target's prototype frame, a the target's covariant
+ * basis at the same point. */
+static void ecs_vec_tm(int e, int i, int j, double hx, double tm[4])
+{
+    const int bsign = (e == 0 || e == 2) ? 1 : -1, shift = e >= 2;
+    const double al = -0.25 * PI + (i - 1.0) * hx, be = bsign * (0.25 * PI - (j - 1.0) * hx);
+    const double as = shift ? be : al, bs = shift ? al : be;
+    double a1[3], a2[3], b1[3], b2[3];
+    ecs_basis(as, bs, a1, a2, b1, b2);
+    const double X = tan(as), Y = tan(bs), sg = sqrt(1 + X * X + Y * Y);
+    const double r0[3] = {X / sg, Y / sg, 1.0 / sg};
+    /* the edge's rotation of the prototype frame: out_k = column k . v */
+    double rb1[3], rb2[3], r[3];
+    const double *in[3] = {b1, b2, r0};
+    double *out[3] = {rb1, rb2, r};
+    for (int m = 0; m < 3; m++) {
+        const double *v = in[m];
+        double *o = out[m];
+        if (e == 0) { o[0] = v[0]; o[1] = -v[2]; o[2] = v[1]; }
+        else if (e == 1) { o[0] = v[0]; o[1] = v[2]; o[2] = -v[1]; }
+        else if (e == 2) { o[0] = -v[2]; o[1] = v[1]; o[2] = v[0]; }
+        else { o[0] = v[2]; o[1] = v[1]; o[2] = -v[0]; }
+    }
+    double A1[3], A2[3], B1[3], B2[3];
+    ecs_basis(atan(r[0] / r[2]), atan(r[1] / r[2]), A1, A2, B1, B2);
+    tm[0] = rb1[0] * A1[0] + rb1[1] * A1[1] + rb1[2] * A1[2];
+    tm[1] = rb2[0] * A1[0] + rb2[1] * A1[1] + rb2[2] * A1[2];
+    tm[2] = rb1[0] * A2[0] + rb1[1] * A2[1] + rb1[2] * A2[2];
+    tm[3] = rb2[0] * A2[0] + rb2[1] * A2[1] + rb2[2] * A2[2];
+}
+
+/* The winds on the lat-lon grid as the shallow-water output writes u.dat and
+ * v.dat (swm_output_diag_mod.f90 write_fields L75-76, latlon_vec_write
+ * latlon_outputer_mod.f90 L65-107, do_latlon_vector_regrid
+ * latlon_regrid_mod.f90 L166-262): the covariant u, v colocated at the xy
+ * points by interp2d_uv2pvec_Ch_sbp42 (i2c -> c2i interpolation along x for
+ * u, along y for v, the 42 closure whatever the operator's order), the
+ * exchanged vector halo turned into the target's extended components
+ * (ecs_halo_vec_xy_mod.f90 L60-130), the scalar ECS procedure on each
+ * component, cubic interpolation, and the native components turned into
+ * zonal / meridional ones by q = [iv.b1, iv.b2; jv.b1, jv.b2]
+ * (init_transform_to_latlon_vec_tile, regrid_factory_mod.f90 L338-382; b
+ * the panel's contravariant basis, iv / jv the lat-lon unit vectors).  Float
+ * outputs as latlon_vec_write stores them. */
+void swo_latlon_uv(swo *s, const double *u, const double *v, int nlat, int nlon, float *uo, float *vo)
+{
+    const int N = s->N, n = N + 1, hw = 2;
+    const double hx = 0.5 * PI / N;
+    fld EU, EV;
+    fld_init(&EU, n, n);
+    fld_init(&EV, n, n);
+    double *buf = malloc((size_t)(n + 2 * HALO) * sizeof(double));
+    for (int p = 0; p < 6; p++) {
+        /* (i_c2i reads and writes 1-based lines given by their index-0 address) */
+        for (int j = 1; j <= n; j++) /* u rows: N centres -> N + 1 points */
+            i_c2i(42, u + ((size_t)p * n + (j - 1)) * N - 1, 1, N, AT(&EU, p, 0, j), 1);
+        for (int i = 1; i <= n; i++) { /* v columns */
+            for (int j = 1; j <= N; j++) buf[j] = v[((size_t)p * N + (j - 1)) * n + (i - 1)];
+            i_c2i(42, buf, 1, N, AT(&EV, p, i, 0), EV.sx);
+        }
+    }
+    free(buf);
+    hmap m = {0, 0};
+    build_xy_map(&m, N, hw + 1, 1);
+    apply_map(&m, &EU, &EV);
+    free(m.e);
+    for (int p = 0; p < 6; p++) {
+        for (int e = 0; e < 4; e++)
+            for (int j = 1; j <= hw + 1; j++)
+                for (int a = 1; a <= n; a++) {
+                    double tm[4];
+                    ecs_vec_tm(e, a, j, hx, tm);
+                    if (g_mutation == 14) { /* 14: the components left in the neighbour's frame */
+                        tm[0] = tm[3] = 1.0;
+                        tm[1] = tm[2] = 0.0;
+                    }
+                    const int ti = e == 0 ? a : e == 1 ? a : e == 2 ? 1 - j : n + j;
+                    const int tj = e == 0 ? 1 - j : e == 1 ? n + j : a;
+                    double *pu = AT(&EU, p, ti, tj), *pv = AT(&EV, p, ti, tj);
+                    const double zu = *pu, zv = *pv;
+                    *pu = tm[0] * zu + tm[1] * zv;
+                    *pv = tm[2] * zu + tm[3] * zv;
+                }
+        ecs_xy_halo_panel(&EU, p, n, hx, hw);
+        ecs_xy_halo_panel(&EV, p, n, hx, hw);
+    }
+    for (int j = 1; j <= nlat; j++) {
+        const double phi = -0.5 * PI + (j - 1) * PI / (nlat - 1.0);
+        for (int i = 1; i <= nlon; i++) {
+            const double lam = (i - 1) * 2.0 * PI / nlon;
+            const double r[3] = {cos(phi) * cos(lam), cos(phi) * sin(lam), sin(phi)};
+            int pan = 0;
+            double rn = 0.0;
+            for (int q = 0; q < 6; q++) {
+                const double d = -(r[0] * T_N[q][0] + r[1] * T_N[q][1] + r[2] * T_N[q][2]);
+                if (d > rn) {
+                    pan = q;
+                    rn = d;
+                }
+            }
+            const double al = atan((T_EX[pan][0] * r[0] + T_EX[pan][1] * r[1] + T_EX[pan][2] * r[2]) / rn);
+            const double be = atan((T_EY[pan][0] * r[0] + T_EY[pan][1] * r[1] + T_EY[pan][2] * r[2]) / rn);
+            double zx = (al + 0.25 * PI) / hx + 1.0, zy = (be + 0.25 * PI) / hx + 1.0;
+            const int xi = (int)zx, yi = (int)zy;
+            zx -= xi;
+            zy -= yi;
+            const double wx[4] = {-zx * (zx - 1.0) * (zx - 2.0) / 6.0, (zx + 1.0) * (zx - 1.0) * (zx - 2.0) / 2.0,
+                                  -(zx + 1.0) * zx * (zx - 2.0) / 2.0, (zx + 1.0) * zx * (zx - 1.0) / 6.0};
+            const double wy[4] = {-zy * (zy - 1.0) * (zy - 2.0) / 6.0, (zy + 1.0) * (zy - 1.0) * (zy - 2.0) / 2.0,
+                                  -(zy + 1.0) * zy * (zy - 2.0) / 2.0, (zy + 1.0) * zy * (zy - 1.0) / 6.0};
+            double vu = 0.0, vv = 0.0;
+            for (int b = -1; b <= 2; b++)
+                for (int a = -1; a <= 2; a++) {
+                    vu = vu + wx[a + 1] * wy[b + 1] * *AT(&EU, pan, xi + a, yi + b);
+                    vv = vv + wx[a + 1] * wy[b + 1] * *AT(&EV, pan, xi + a, yi + b);
+                }
+            double a1[3], a2[3], b1[3], b2[3], B1[3], B2[3];
+            ecs_basis(al, be, a1, a2, b1, b2);
+            proto2real(pan, b1, B1);
+            proto2real(pan, b2, B2);
+            const double iv[3] = {-sin(lam), cos(lam), 0.0};
+            const double jv[3] = {-sin(phi) * cos(lam), -sin(phi) * sin(lam), cos(phi)};
+            const double q11 = iv[0] * B1[0] + iv[1] * B1[1] + iv[2] * B1[2];
+            const double q12 = iv[0] * B2[0] + iv[1] * B2[1] + iv[2] * B2[2];
+            const double q21 = jv[0] * B1[0] + jv[1] * B1[1] + jv[2] * B1[2];
+            const double q22 = jv[0] * B2[0] + jv[1] * B2[1] + jv[2] * B2[2];
+            const size_t k = (size_t)(j - 1) * nlon + (i - 1);
+            uo[k] = (float)(q11 * vu + q12 * vv);
+            vo[k] = (float)(q21 * vu + q22 * vv);
+        }
+    }
+    fld_free(&EU);
+    fld_free(&EV);
+}
+
 /* test_fields_mod.f90 generate_gaussian_hill_scalar_field L491-514 (times amp) */
 void swo_gaussian_hill(const swo *s, double lam0, double phi0, double d0,
                        double amp, double *h)

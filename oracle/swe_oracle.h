@@ -78,6 +78,9 @@ void swo_solve(swo *s, const double *ru, const double *rv, const double *rh, dou
                double *u, double *v, double *h, int *iters, double *relres);
 /* print_CFL_diag: gravity-wave CFL, advective CFL along x and y */
 void swo_cfl_diag(swo *s, const double *u, const double *v, const double *h, double dt, double grav, double out[3]);
+/* the covariant winds on the lat-lon grid as the output writes u.dat and
+ * v.dat: zonal uo, meridional vo (float, longitude fastest) */
+void swo_latlon_uv(swo *s, const double *u, const double *v, int nlat, int nlon, float *uo, float *vo);
 /* print_integral_tends of the operator's tendency: out = TE_tend,
  * Enstrophy_tend, then the sums of their terms' absolute values */
 void swo_integral_tends(swo *s, const double *u, const double *v, const double *h, double grav, double out[4]);

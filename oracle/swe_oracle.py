@@ -56,6 +56,8 @@ def _load():
     lib.swo_integral_tends.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
     lib.swo_solve.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                               _dp, _dp, _dp, _ip, _dp]
+    lib.swo_latlon_uv.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
     lib.swo_cfl_diag.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
     lib.swo_latlon_h.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     lib.swo_gaussian_hill.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]
@@ -254,6 +256,18 @@ class Oracle:
         out = np.zeros((nlat, nlon), np.float32)
         self.lib.swo_latlon_h(self._h, _p(h), nlat, nlon, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
         return out
+
+    def latlon_uv(self, u, v, nlat=None, nlon=None):
+        """The covariant winds on the lat-lon grid as the shallow-water
+        output writes u.dat / v.dat: float32 (zonal, meridional), each
+        (nlat, nlon)."""
+        nlat = 2 * self.N + 1 if nlat is None else nlat
+        nlon = 4 * self.N if nlon is None else nlon
+        u, v = self._c(u), self._c(v)
+        uo, vo = np.zeros((nlat, nlon), np.float32), np.zeros((nlat, nlon), np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        self.lib.swo_latlon_uv(self._h, _p(u), _p(v), nlat, nlon, uo.ctypes.data_as(fp), vo.ctypes.data_as(fp))
+        return uo, vo
 
     def gaussian_hill(self, lam0, phi0, d0, amp):
         h = self.zeros(MESH_XY)

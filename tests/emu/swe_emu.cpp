@@ -21,7 +21,7 @@
 // multi-instance launches of ensembles, the tile-wise integrals).
 //
 // usage: swe_emu <case> N order prec [steps] [seed] [flags]
-//   case  : apply | rk4 | integrals | diag | tends | latlon | multi
+//   case  : apply | rk4 | integrals | diag | tends | latlon | latlonuv | multi
 //   prec  : 64 | 32
 //   flags : bit 0 every tile on the general path, bit 1 plain block order,
 //           bit 2 compensated summation (rk4)
@@ -523,6 +523,33 @@ struct Plan {
         std::copy(o.begin(), o.end(), out);
     }
 
+    // Engine::latlon_uv: colocation, the two components' extended arrays,
+    // the interpolation and the zonal / meridional turn
+    void latlon_uv(const T *st, int nlat, int nlon, float *uo, float *vo)
+    {
+        swe::LatlonTables t;
+        std::string err;
+        if (swe::build_latlon_vec_tables(N, nlat, nlon, t, err)) {
+            std::fprintf(stderr, "%s\n", err.c_str());
+            std::exit(2);
+        }
+        const int n = N + 1;
+        const long nxy = 6L * n * n;
+        const int nr = (int)t.rp.size() - 1, np = (int)t.base.size();
+        std::vector<double> uv(2 * nxy, NAN), ext(nr, NAN);
+        std::vector<float> o(2 * (size_t)np, NAN);
+        emu::launch_flat((unsigned)((nxy + 255) / 256), 256, [&] { k_ll_colocate<T>(d, st, st + d.F, uv.data()); });
+        emu::launch_flat((nr + 255) / 256, 256, [&] {
+            k_ll_ext<double>(uv.data(), t.rp.data(), t.src.data(), t.w.data(), nr, ext.data());
+        });
+        emu::launch_flat((np + 255) / 256, 256, [&] {
+            k_ll_interp_uv(ext.data(), nr / 2, t.base.data(), t.fx.data(), t.fy.data(), t.q.data(), np, t.ew, o.data(),
+                           o.data() + np);
+        });
+        std::copy(o.begin(), o.begin() + np, uo);
+        std::copy(o.begin() + np, o.end(), vo);
+    }
+
     // Engine::integral_tends: apply into a tendency state, then the tile sums
     template <int O>
     void tends(T *st, T *tend, double grav, double out[2])
@@ -657,6 +684,30 @@ int run_case(const std::string &cs, int N, int steps, unsigned seed, int flags)
         }
         std::printf("latlon %d x %d max ulp difference %d (bar 1)\n", nlat, nlon, ulps);
         bad = ulps > 1;
+    } else if (cs == "latlonuv") {
+        T *st = pl.state();
+        pl.upload(st, s0.u.data(), s0.v.data(), s0.h.data());
+        const int nlat = 2 * N + 1, nlon = 4 * N;
+        const size_t np = (size_t)nlat * nlon;
+        std::vector<float> gu(np), gv(np), ru(np), rv(np);
+        pl.latlon_uv(st, nlat, nlon, gu.data(), gv.data());
+        std::vector<double> uu = s0.u, vv = s0.v;
+        if (sizeof(T) == 4) {
+            for (double &x : uu) x = (double)(float)x;
+            for (double &x : vv) x = (double)(float)x;
+        }
+        swo_latlon_uv(o, uu.data(), vv.data(), nlat, nlon, ru.data(), rv.data());
+        // the frame change of the halo is derived differently here and in
+        // the oracle: FP64 sums that differ in their last bits, rounded to
+        // float and turned into zonal / meridional components
+        double mx = 0, e = 0;
+        for (size_t k = 0; k < np; k++) {
+            mx = std::max(mx, (double)std::max(std::fabs(ru[k]), std::fabs(rv[k])));
+            e = std::max(e, (double)std::max(std::fabs(gu[k] - ru[k]), std::fabs(gv[k] - rv[k])));
+            if (!std::isfinite(gu[k]) || !std::isfinite(gv[k])) e = INFINITY;
+        }
+        std::printf("latlonuv %d x %d max error / max value %.3e (bar 4e-7)\n", nlat, nlon, e / mx);
+        bad = !(e <= 4e-7 * mx);
     } else if (cs == "tends") {
         T *st = pl.state(), *tend = pl.state();
         Fields s1 = s0;
@@ -778,7 +829,7 @@ int dispatch(const std::string &cs, int N, int order, int steps, unsigned seed, 
 int main(int argc, char **argv)
 {
     if (argc < 5) {
-        std::fprintf(stderr, "usage: %s apply|rk4|integrals|diag|tends|latlon|multi N order 64|32 [steps] [seed] [flags]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s apply|rk4|integrals|diag|tends|latlon|latlonuv|multi N order 64|32 [steps] [seed] [flags]\n", argv[0]);
         return 2;
     }
     const std::string cs = argv[1];

@@ -43,6 +43,8 @@ CASES = [
     ("tends", 57, 42, 32, 1, 21, 0),
     ("latlon", 12, 42, 64, 1, 22, 0),   # lat-lon output of h (ECS halo map, cubic interpolation)
     ("latlon", 17, 21, 32, 1, 23, 0),
+    ("latlonuv", 12, 42, 64, 1, 24, 0),  # lat-lon winds (colocation, vector halo map, zonal / meridional)
+    ("latlonuv", 17, 21, 32, 1, 25, 0),
 ]
 
 

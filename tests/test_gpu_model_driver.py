@@ -84,6 +84,14 @@ def test_gauss_config_matches_oracle(oracle_mod, tmp_path, order, precision, tol
         diff = np.abs(hd[k].view(np.int32).astype(np.int64) - llref.view(np.int32).astype(np.int64))
         scale = np.abs(llref).max()
         assert np.all((diff <= 1) | (np.abs(hd[k] - llref) <= 1e-11 * scale)), (k, diff.max())
+    # u.dat / v.dat: the winds of the same states (at rest at t = 0)
+    ud = np.fromfile(tmp_path / "u.dat", dtype=np.float32).reshape(3, 65, 128)
+    vd = np.fromfile(tmp_path / "v.dat", dtype=np.float32).reshape(3, 65, 128)
+    assert not np.any(ud[0]) and not np.any(vd[0])
+    if precision.startswith("fp64"):
+        ru, rv = o.latlon_uv(ref[0], ref[1])
+        mx = max(np.abs(ru).max(), np.abs(rv).max())
+        assert np.abs(ud[2] - ru).max() <= 1e-6 * mx and np.abs(vd[2] - rv).max() <= 1e-6 * mx
     # print_integral_tends's line after each diagnostics line (TE_tend only:
     # the enstrophy tendency divides by the perturbation h as well)
     tl = re.findall(r"dynamics: TE_tend = (.*), Enstrophy_tend = (.*)", stdout)

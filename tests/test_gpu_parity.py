@@ -461,6 +461,28 @@ def test_latlon_h_matches_oracle(oracle_mod, order, N, precision, grid):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("order,N,precision,grid", LATLON_CASES)
+def test_latlon_uv_matches_oracle(oracle_mod, order, N, precision, grid):
+    """swe_latlon_uv (u.dat / v.dat: colocation, the vector ECS halo, cubic
+    regrid, zonal / meridional turn) against the oracle on random winds:
+    the halo's frame change is derived differently on each side, so the
+    float outputs agree to 4e-7 of the largest value, not bitwise."""
+    _say(f"latlon_uv order={order} N={N} precision={precision} grid={grid}")
+    o = oracle_mod.Oracle(N, order)
+    u, v, h = _random_state(o, 90 + N)
+    op = _op(N, order, precision)
+    st = op.create_stvec().upload(u, v, h)
+    nlat, nlon = grid if grid else (2 * N + 1, 4 * N)
+    if precision == cs.FP32:
+        u, v = (a.astype(np.float32).astype(np.float64) for a in (u, v))
+    for a, b in ((nlat, nlon), (nlat + 2, nlon + 5)):
+        gu, gv = op.latlon_uv(st, a, b)
+        ru, rv = o.latlon_uv(u, v, a, b)
+        mx = max(np.abs(ru).max(), np.abs(rv).max())
+        assert np.abs(gu - ru).max() <= 4e-7 * mx and np.abs(gv - rv).max() <= 4e-7 * mx
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("order,N,precision,omega", DIAG_CASES)
 def test_integral_tends_match_oracle(oracle_mod, order, N, precision, omega):
     """swe_integral_tends (print_integral_tends's TE_tend and Enstrophy_tend

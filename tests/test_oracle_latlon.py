@@ -9,7 +9,10 @@ The checks: a smooth function on the sphere converges at fourth order to
 its exact lat-lon values (any wrong panel, orientation, row or weight
 breaks this); lat-lon points that fall on grid nodes return the node value;
 the output depends on the copies of an edge or corner point only through
-their sum (the averaging); constants are reproduced."""
+their sum (the averaging); constants are reproduced.  The winds (u.dat,
+v.dat: swo_latlon_uv) of solid rotation converge to the exact zonal and
+meridional components, and a planted mistake in the vector halo's frame
+change breaks that."""
 import numpy as np
 import pytest
 
@@ -149,3 +152,58 @@ def test_planted_mistakes_break_convergence(oracle_mod, mutation):
     finally:
         oracle_mod.set_mutation(0)
     assert err > 5e-3
+
+
+def _rotation_winds(o, u0, axis):
+    """Covariant components of V = u0 axis x r on the u and v points."""
+    from oracle.swe_oracle import MESH_U, MESH_V
+    ax = np.asarray(axis, float) / np.linalg.norm(axis)
+    comps = []
+    for m in (MESH_U, MESH_V):
+        a1, a2 = o.basis(m)
+        r = np.cross(a1, a2)
+        r /= np.linalg.norm(r, axis=-1, keepdims=True)
+        V = u0 * np.cross(ax, r)
+        comps.append(np.einsum("...k,...k->...", V, a1 if m == MESH_U else a2).reshape(o.shape(m)))
+    return comps
+
+
+def _exact_latlon_winds(N, u0, axis):
+    nlat, nlon = 2 * N + 1, 4 * N
+    lam = np.broadcast_to(np.arange(nlon) * 2 * np.pi / nlon, (nlat, nlon))
+    phi = np.broadcast_to((-0.5 * np.pi + np.arange(nlat) * np.pi / (nlat - 1))[:, None], (nlat, nlon))
+    V = u0 * np.cross(np.asarray(axis, float) / np.linalg.norm(axis), latlon_points(nlat, nlon))
+    iv = np.stack([-np.sin(lam), np.cos(lam), 0 * lam], -1)
+    jv = np.stack([-np.sin(phi) * np.cos(lam), -np.sin(phi) * np.sin(lam), np.cos(phi)], -1)
+    return np.einsum("...k,...k->...", V, iv), np.einsum("...k,...k->...", V, jv)
+
+
+def test_winds_converge(oracle_mod):
+    """u.dat / v.dat of solid rotation about a tilted axis: the zonal and
+    meridional winds converge to their exact lat-lon values at second order
+    (the colocating c2i interpolation's closure), everywhere including the
+    poles, the cube edges and corners (the vector halo's transform)."""
+    u0, axis = 20.0, (0.3, -0.5, 0.8)
+    errs = []
+    for N in (16, 32, 64):
+        o = oracle_mod.Oracle(N, 42)
+        uo, vo = o.latlon_uv(*_rotation_winds(o, u0, axis))
+        ez, em = _exact_latlon_winds(N, u0, axis)
+        assert uo.dtype == np.float32 and uo.shape == (2 * N + 1, 4 * N)
+        errs.append(max(np.abs(uo - ez).max(), np.abs(vo - em).max()) / u0)
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert errs[-1] < 3e-4 and np.all(rates > 1.7), (errs, rates)
+
+
+def test_winds_planted_halo_frame_mistake(oracle_mod):
+    """Leaving the exchanged components in the neighbour's frame (mutation
+    14) puts errors of the size of the wind near the cube edges."""
+    u0, axis, N = 20.0, (0.3, -0.5, 0.8), 32
+    oracle_mod.set_mutation(14)
+    try:
+        o = oracle_mod.Oracle(N, 42)
+        uo, vo = o.latlon_uv(*_rotation_winds(o, u0, axis))
+    finally:
+        oracle_mod.set_mutation(0)
+    ez, em = _exact_latlon_winds(N, u0, axis)
+    assert max(np.abs(uo - ez).max(), np.abs(vo - em).max()) > 0.1 * u0
