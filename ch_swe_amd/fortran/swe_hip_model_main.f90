@@ -155,7 +155,7 @@ program swe_hip_model
     name = config%get_str("timescheme_name")
     implicit = name == "implicit_Eul" .or. name == "crank_nikolson"
     if (.not. implicit .and. name /= "rk4") &
-        call swe_hip_abort("timescheme_name "//name//": only rk4, implicit_Eul and crank_nikolson run here")
+        call swe_hip_abort("timescheme_name = '"//name//"': only rk4, implicit_Eul and crank_nikolson run here")
     if (implicit) then
         call expect_str(opsec//"%helm_solver_cfg%solver_name", "", "cg")
         implicit_ts = create_implicit_hip(name == "crank_nikolson", config%get_int(opsec//"%helm_solver_cfg%max_iter"), &
