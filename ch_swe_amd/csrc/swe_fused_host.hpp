@@ -81,12 +81,11 @@ inline int build_fused_tables(int N, int TY, FusedTables &ft, std::string &err, 
     ft.segs.clear();
     for (int p = 0; p < 6; p++)
         for (int side = 0; side < 4; side++) {
-            // records segments: uniform pieces of at most TY (sides 0, 1) or
-            // F_TX (sides 2, 3) points, the extents the band tiles fit in LDS
-            int cnt = side <= 1 ? (n + TY - 1) / TY : (n + F_TX - 1) / F_TX;
+            // records segments: uniform pieces of at most R_SEG points, the
+            // extents a records frame holds
+            const int cnt = (n + R_SEG - 1) / R_SEG;
             for (int k = 0; k < cnt; k++) {
-                int lo = side <= 1 ? 1 + k * TY : 1 + k * F_TX;
-                int hi = std::min(n, side <= 1 ? lo + TY - 1 : lo + F_TX - 1);
+                const int lo = 1 + k * R_SEG, hi = std::min(n, lo + R_SEG - 1);
                 ft.segs.push_back(make_int4(p, side, lo, hi));
             }
         }
