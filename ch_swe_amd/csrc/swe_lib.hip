@@ -287,23 +287,48 @@ __global__ void k_dot_final(const double *part, int nb, double *out)
     *out = s;
 }
 
+__global__ void k_fill(double *p, int n, double v)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+// The convergence state lives on the device, so that the host reads it
+// only every few iterations: st[0] = 1 once converged, st[1] = the
+// iteration that met the tolerance, st[2] = the latest relative residual.
+// After convergence the update kernels do nothing, so x stays the iterate
+// that met the tolerance whatever iterations follow in the same batch.
+
 // x += alpha p, r -= alpha Ap, alpha = |r|^2 / (p, Ap) with |r| = sqrt(*rr)
 template <typename T>
-__global__ void k_cg_step(long n, T *x, T *r, const T *p, const T *Ap, const double *rr, const double *pap)
+__global__ void k_cg_step(long n, T *x, T *r, const T *p, const T *Ap, const double *rr, const double *pap,
+                          const double *st)
 {
     const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    if (k >= n || st[0] != 0.0) return;
     const double rn = sqrt(*rr), alpha = rn * rn / *pap;
     x[k] = x[k] + T(alpha) * p[k];
     r[k] = r[k] + T(-alpha) * Ap[k];
 }
 
+// the relative residual of iteration it against the tolerance (one thread)
+__global__ void k_cg_check(const double *rr_new, const double *bb, double rel_tol, int it, double *st)
+{
+    if (threadIdx.x != 0 || st[0] != 0.0) return;
+    const double rel = sqrt(*rr_new) / sqrt(*bb);
+    st[2] = rel;
+    if (rel < rel_tol) {
+        st[0] = 1.0;
+        st[1] = it;
+    }
+}
+
 // p = r + beta p, beta = (|r_new| / |r_old|)^2
 template <typename T>
-__global__ void k_cg_dir(long n, T *p, const T *r, const double *rr_new, const double *rr_old)
+__global__ void k_cg_dir(long n, T *p, const T *r, const double *rr_new, const double *rr_old, const double *st)
 {
     const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    if (k >= n || st[0] != 0.0) return;
     const double q = sqrt(*rr_new) / sqrt(*rr_old), beta = q * q;
     p[k] = T(1.0) * r[k] + T(beta) * p[k];
 }
@@ -989,6 +1014,7 @@ struct Engine : EngineBase {
       *sv_b = nullptr;
     double *sv_part = nullptr, *sv_sc = nullptr;
     static constexpr int DOT_BLOCKS = 256;
+    static constexpr int CG_BATCH = 8;  // CG iterations between convergence read-backs
     void xy_dot(const T *a, const T *b, double *out)
     {
         if (order == 21) k_xy_dot<T, 21><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, geo.hx, a, b, sv_part);
@@ -1041,19 +1067,30 @@ struct Engine : EngineBase {
         int it = 0;
         if (!(rel < rel_tol)) {
             HIP_TRY(hipMemcpyAsync(sv_p, sv_r, F * sizeof(T), hipMemcpyDeviceToDevice, pl->stream));
+            double *cst = sv_sc + 4;  // convergence state (k_cg_check)
+            k_fill<<<1, 64, 0, pl->stream>>>(cst, 3, 0.0);
             int a = 0, b = 2;  // slots of |r|^2 before and after the step
-            for (it = 1; it <= max_iter; it++) {
+            bool done = false;
+            it = max_iter + 1;  // not converged (also when max_iter is 0)
+            for (int k = 1; k <= max_iter && !done; k++) {
                 if ((rc = helm(sv_p, sv_ap, dt))) return rc;
                 xy_dot(sv_p, sv_ap, sv_sc + 1);
-                k_cg_step<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, x, sv_r, sv_p, sv_ap, sv_sc + a, sv_sc + 1);
+                k_cg_step<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, x, sv_r, sv_p, sv_ap, sv_sc + a, sv_sc + 1,
+                                                                     cst);
                 xy_dot(sv_r, sv_r, sv_sc + b);
-                double rrn;
-                HIP_TRY(hipMemcpyAsync(&rrn, sv_sc + b, sizeof(double), hipMemcpyDeviceToHost, pl->stream));
-                HIP_TRY(hipStreamSynchronize(pl->stream));
-                rel = std::sqrt(rrn) / bn;
-                if (rel < rel_tol) break;
-                k_cg_dir<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, sv_p, sv_r, sv_sc + b, sv_sc + a);
+                k_cg_check<<<1, 64, 0, pl->stream>>>(sv_sc + b, sv_sc + 3, rel_tol, k, cst);
+                k_cg_dir<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, sv_p, sv_r, sv_sc + b, sv_sc + a, cst);
                 std::swap(a, b);
+                // the host looks at the convergence state every CG_BATCH
+                // iterations and after the last
+                if (k % CG_BATCH == 0 || k == max_iter) {
+                    double cs[3];
+                    HIP_TRY(hipMemcpyAsync(cs, cst, 3 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+                    HIP_TRY(hipStreamSynchronize(pl->stream));
+                    done = cs[0] != 0.0;
+                    rel = cs[2];
+                    it = done ? (int)cs[1] : k + 1;
+                }
             }
         }
         if (iters) *iters = it;

@@ -120,8 +120,8 @@ def test_solve_fp32_plan(oracle_mod):
 
 def test_solve_edge_cases():
     """A zero right-hand side returns zero at once; a too small max_iter
-    reports max_iter + 1 iterations and a residual above the tolerance;
-    vout aliasing rhs is refused."""
+    (0 included) reports max_iter + 1 iterations and a residual above the
+    tolerance; vout aliasing rhs is refused."""
     _say("solve edge cases")
     N = 12
     op = cs.create_linear_swm_operator(N, 21, cs.FP64)
@@ -135,5 +135,7 @@ def test_solve_edge_cases():
                                  rng.standard_normal((6, N + 1, N + 1)))
     it, rel = op.solve(x, r, 5000.0, rel_tol=1e-14, max_iter=3)
     assert it == 4 and rel >= 1e-14
+    it, rel = op.solve(x, r, 5000.0, rel_tol=1e-14, max_iter=0)
+    assert it == 1 and rel >= 1e-14
     with pytest.raises(cs.SweError):
         op.solve(r, r, 1000.0)
