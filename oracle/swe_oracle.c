@@ -295,11 +295,66 @@ static void apply_map(const hmap *m, fld *a, fld *b)
  * sbp_interp_i2c_42_mod.f90 L18-19 */
 static const double C13 = 0.1264926207034815, C14 = -0.029892648949074313;
 
+/* sbp63 (sbp_diff_i2c_63_mod.f90 L11-75, sbp_diff_c2i_63_mod.f90 L13-60,
+ * sbp_interp_i2c_63_mod.f90 L14-90, sbp_interp_c2i_63_mod.f90 L11-62): the
+ * closure rows evaluated to doubles (row r reads points 1..9 of its end;
+ * the right end mirrors them), the interior stencils, the centre and
+ * interface norms' boundary weights and the SAT projection. */
+static const double D63_I2C[6][9] = {{-0.9795214020218487, 0.9597522747540616, -0.0021284121310923276, 0.04308560808739484, -0.02118806868851538, 0.0, 0.0, 0.0, 0.0},
+ {0.10094185622089052, -1.3621007582168956, 1.480651137325344, -0.2787674248835621, 0.059275189554223895, 0.0, 0.0, 0.0, 0.0},
+ {0.1050522197237869, -0.3755418649660661, -0.5066880707069372, 0.7227932046793403, 0.05138416400746168, 0.0030003472624146313, 0.0, 0.0, 0.0},
+ {-0.010281664659776757, 0.009016888173475901, 0.1158762991569453, -1.3063741185003286, 1.2879206487212211, -0.10361859147794651, 0.007460538586412149, 0.0, 0.0},
+ {-0.08522272532774755, 0.2300959768599209, -0.06748505183821596, -0.28075281972041166, -0.77959568679678, 1.0363920760788554, -0.05757733755993641, 0.0041455683043154215, 0.0},
+ {0.03958146265158008, -0.10105244572213268, 0.0269485128845091, 0.10551623486389128, -0.009407429883264403, -1.1945625283432082, 1.1945625283432082, -0.066364584907956, 0.004778250113372832}};
+static const double D63_C2I[5][9] = {{-3.137061163091724, 6.472195962306311, -4.476505551641335, 0.9669525120033824, 0.2921335971506185, -0.11771535672725233, 0.0, 0.0, 0.0},
+ {-0.8550973738466245, 0.5371097071174838, 0.4442015766629761, -0.00428923422758698, -0.1969782207705677, 0.07505354506431924, 0.0, 0.0, 0.0},
+ {0.003267033496606009, -1.0058841165930086, 1.032532798072641, -0.09496402962593153, 0.0995309639229438, -0.03448264927325102, 0.0, 0.0, 0.0},
+ {-0.0460358301286618, 0.1318266207921457, -1.0252815152152988, 0.7452431221796392, 0.2882309687613436, -0.093983366389168, 0.0, 0.0, 0.0},
+ {0.025410671711714995, -0.031462633326690415, -0.08181244708002722, -0.8246706459806198, 0.8983528780132024, 0.009405106598725824, 0.004777070063694267, 0.0, 0.0}};
+static const double I63_I2C[6][9] = {{0.4559762640735722, 0.5156609261411327, 0.09215963713516839, -0.055207108988023945, -0.008589718361849435, 0.0, 0.0, 0.0, 0.0},
+ {-0.2329150785274282, 1.0886403314451474, 0.006569476829126231, 0.1526003661160169, -0.014895095862863417, 0.0, 0.0, 0.0, 0.0},
+ {-0.13300622901536394, 0.24335188147388265, 0.43548086151449933, 0.4385084167668463, 0.008164201104099517, 0.007500868156036579, 0.0, 0.0, 0.0},
+ {0.10717671074312715, -0.3332211159670528, 0.3310769312612241, 0.29067744812134255, 0.7410665665922483, -0.15542788721691977, 0.018651346466030373, 0.0, 0.0},
+ {0.09305432569823871, -0.07099703081266313, -0.291616405335888, 0.3005623879587947, 0.5268027700312061, 0.5181960380394277, -0.08636600633990461, 0.010363920760788554, 0.0},
+ {-0.05115521553501697, 0.05753938634775091, 0.09801308881771609, -0.1230378129758785, -0.08832072291927771, 0.5972812641716041, 0.5972812641716041, -0.09954687736193402, 0.011945625283432081}};
+static const double I63_C2I[5][9] = {{1.6267303202606325, -0.3677632041998704, -0.6299565857805535, 0.2041457799345569, 0.3189794129688649, -0.15213572318363067, 0.0, 0.0, 0.0},
+ {0.4594313713417274, 0.42927755971902565, 0.287843512318841, -0.15850960868795075, -0.060778415165360984, 0.04273558047371772, 0.0, 0.0, 0.0},
+ {0.14146161692901638, 0.004462990795171873, 0.8874262064612957, 0.2713272665549153, -0.43009320031948844, 0.12541511957908888, 0.0, 0.0, 0.0},
+ {-0.058987332524401276, 0.07216334765484655, 0.6220237974938234, 0.16582184683346107, 0.3085681858542267, -0.10958984531195667, 0.0, 0.0, 0.0},
+ {-0.0103015766372022, -0.007906156741522508, 0.012998815562764554, 0.47451358497362967, 0.607051568675785, -0.0882989109926898, 0.01194267515923567, 0.0, 0.0}};
+static const double H63_C[6] = {325363.0 / 276480, 144001.0 / 276480, 43195.0 / 27648,
+                                86857.0 / 138240, 312623.0 / 276480, 271229.0 / 276480};
+static const double H63_I[5] = {95.0 / 288, 317.0 / 240, 23.0 / 30, 793.0 / 720, 157.0 / 160};
+static const double R63[4] = {35.0 / 16, -35.0 / 16, 21.0 / 16, -5.0 / 16};
+static const double QD63[3] = {75.0 / 64, -25.0 / 384, 3.0 / 640};
+static const double QI63[3] = {75.0 / 128, -25.0 / 256, 3.0 / 256};
+
 /* interfaces -> centres derivative, n centres out, reads f[0..n+2]
  * (f[0], f[n+2] = neighbour's duplicates of the edge points)
  * sbp_diff_i2c_21_mod.f90 L24-144, sbp_diff_i2c_42_mod.f90 L29-274 */
 static void d_i2c(int order, const double *f, ptrdiff_t fs, int n, double *o, ptrdiff_t os)
 {
+    if (order == 63) {
+        for (int r = 0; r < 6; r++) {
+            double sl = 0, sr = 0;
+            for (int k = 0; k < 9; k++) {
+                sl += D63_I2C[r][k] * L1(f, fs, 1 + k);
+                sr += D63_I2C[r][k] * L1(f, fs, n + 1 - k);
+            }
+            L1(o, os, 1 + r) = sl;
+            L1(o, os, n - r) = -sr;
+        }
+        for (int k = 7; k <= n - 6; k++)
+            L1(o, os, k) = QD63[0] * (L1(f, fs, k + 1) - L1(f, fs, k)) + QD63[1] * (L1(f, fs, k + 2) - L1(f, fs, k - 1)) +
+                           QD63[2] * (L1(f, fs, k + 3) - L1(f, fs, k - 2));
+        if (g_no_sat) return;
+        const double jl = L1(f, fs, 1) - L1(f, fs, 0), jr = L1(f, fs, n + 2) - L1(f, fs, n + 1);
+        for (int r = 0; r < 4; r++) {
+            L1(o, os, 1 + r) += 0.5 * R63[r] / H63_C[r] * jl;
+            L1(o, os, n - r) += 0.5 * R63[r] / H63_C[r] * jr;
+        }
+        return;
+    }
     if (order == 21) {
         for (int k = 1; k <= n; k++) L1(o, os, k) = L1(f, fs, k + 1) - L1(f, fs, k);
         if (g_no_sat) return;
@@ -343,6 +398,30 @@ static void d_i2c(int order, const double *f, ptrdiff_t fs, int n, double *o, pt
 static void d_c2i(int order, const double *f, ptrdiff_t fs, int n, double *o, ptrdiff_t os)
 {
     int nx = n + 1;
+    if (order == 63) {
+        for (int r = 0; r < 5; r++) {
+            double sl = 0, sr = 0;
+            for (int k = 0; k < 9; k++) {
+                sl += D63_C2I[r][k] * L1(f, fs, 1 + k);
+                sr += D63_C2I[r][k] * L1(f, fs, nx - 1 - k);
+            }
+            L1(o, os, 1 + r) = sl;
+            L1(o, os, nx - r) = -sr;
+        }
+        for (int k = 6; k <= nx - 5; k++)
+            L1(o, os, k) = QD63[0] * (L1(f, fs, k) - L1(f, fs, k - 1)) + QD63[1] * (L1(f, fs, k + 1) - L1(f, fs, k - 2)) +
+                           QD63[2] * (L1(f, fs, k + 2) - L1(f, fs, k - 3));
+        if (g_no_sat) return;
+        const double c = 0.5 / H63_I[0];
+        double sl = 0, sr = 0;
+        for (int r = 0; r < 4; r++) {
+            sl += R63[r] * (L1(f, fs, 1 + r) - L1(f, fs, -r));
+            sr += R63[r] * (L1(f, fs, nx - 1 - r) - L1(f, fs, nx + r));
+        }
+        L1(o, os, 1) += c * sl;
+        L1(o, os, nx) -= c * sr;
+        return;
+    }
     if (order == 21) {
         L1(o, os, 1) = L1(f, fs, 2) - L1(f, fs, 1);
         for (int k = 2; k <= nx - 1; k++) L1(o, os, k) = L1(f, fs, k) - L1(f, fs, k - 1);
@@ -386,6 +465,21 @@ static void d_c2i(int order, const double *f, ptrdiff_t fs, int n, double *o, pt
  * sbp_interp_i2c_21_mod.f90 L19-57, sbp_interp_i2c_42_mod.f90 L18-198 */
 static void i_i2c(int order, const double *f, ptrdiff_t fs, int n, double *o, ptrdiff_t os)
 {
+    if (order == 63) {
+        for (int r = 0; r < 6; r++) {
+            double sl = 0, sr = 0;
+            for (int k = 0; k < 9; k++) {
+                sl += I63_I2C[r][k] * L1(f, fs, 1 + k);
+                sr += I63_I2C[r][k] * L1(f, fs, n + 1 - k);
+            }
+            L1(o, os, 1 + r) = sl;
+            L1(o, os, n - r) = sr;
+        }
+        for (int k = 7; k <= n - 6; k++)
+            L1(o, os, k) = QI63[0] * (L1(f, fs, k + 1) + L1(f, fs, k)) + QI63[1] * (L1(f, fs, k + 2) + L1(f, fs, k - 1)) +
+                           QI63[2] * (L1(f, fs, k + 3) + L1(f, fs, k - 2));
+        return;
+    }
     if (order == 21) {
         for (int k = 1; k <= n; k++) L1(o, os, k) = 0.5 * (L1(f, fs, k + 1) + L1(f, fs, k));
         return;
@@ -415,6 +509,21 @@ static void i_i2c(int order, const double *f, ptrdiff_t fs, int n, double *o, pt
 static void i_c2i(int order, const double *f, ptrdiff_t fs, int n, double *o, ptrdiff_t os)
 {
     int nx = n + 1;
+    if (order == 63) {
+        for (int r = 0; r < 5; r++) {
+            double sl = 0, sr = 0;
+            for (int k = 0; k < 9; k++) {
+                sl += I63_C2I[r][k] * L1(f, fs, 1 + k);
+                sr += I63_C2I[r][k] * L1(f, fs, nx - 1 - k);
+            }
+            L1(o, os, 1 + r) = sl;
+            L1(o, os, nx - r) = sr;
+        }
+        for (int k = 6; k <= nx - 5; k++)
+            L1(o, os, k) = QI63[0] * (L1(f, fs, k) + L1(f, fs, k - 1)) + QI63[1] * (L1(f, fs, k + 1) + L1(f, fs, k - 2)) +
+                           QI63[2] * (L1(f, fs, k + 2) + L1(f, fs, k - 3));
+        return;
+    }
     if (order == 21) {
         L1(o, os, 1) = L1(f, fs, 1);
         for (int k = 2; k <= nx - 1; k++) L1(o, os, k) = 0.5 * (L1(f, fs, k) + L1(f, fs, k - 1));
@@ -501,9 +610,13 @@ static void build_geometry(swo *s)
 {
     int N = s->N;
     /* quadrature_factory_mod.f90 L89-188 and sbp_operators_collection_mod.f90 L169-197 */
-    double Ai[5], Ac[5];
+    double Ai[6], Ac[6];
     int ni, nc;
-    if (s->order == 21) {
+    if (s->order == 63) {
+        ni = 5; nc = 6;
+        for (int k = 0; k < 5; k++) Ai[k] = H63_I[k];
+        for (int k = 0; k < 6; k++) Ac[k] = H63_C[k];
+    } else if (s->order == 21) {
         ni = 2; nc = 2;
         Ai[0] = 0.5; Ai[1] = 1;
         Ac[0] = 1; Ac[1] = 1;
@@ -609,9 +722,10 @@ static void build_geometry(swo *s)
 swo *swo_create(int N, int order, double radius, double grav, double h_mean,
                 double omega, const double axis[3], int coriolis_const)
 {
-    if (!(order == 21 || order == 42)) return NULL;
+    if (!(order == 21 || order == 42 || order == 63)) return NULL;
     if (order == 21 && N < 4) return NULL;
     if (order == 42 && N < 8) return NULL;
+    if (order == 63 && N < 12) return NULL;
     swo *s = calloc(1, sizeof(swo));
     s->N = N;
     s->order = order;
@@ -629,7 +743,7 @@ swo *swo_create(int N, int order, double radius, double grav, double h_mean,
     build_geometry(s);
     build_xy_map(&s->m_xy, N, 1, 0);
     build_xy_map(&s->m_xyv, N, 1, 1);
-    build_ch_map(&s->m_ch, N, order == 21 ? 2 : 3);
+    build_ch_map(&s->m_ch, N, order == 21 ? 2 : (order == 42 ? 3 : 4));
     build_tan_map(&s->m_tan, N);
     fld_init(&s->h, N + 1, N + 1);
     fld_init(&s->u, N, N + 1);
@@ -1068,11 +1182,14 @@ static void centre_geom(const swo *s, int p, int i, int j, double *J, double *w,
     const double al = -0.25 * PI + (i - 0.5) * s->hx, be = -0.25 * PI + (j - 0.5) * s->hx;
     const double X = tan(al), Y = tan(be), sg = sqrt(1 + X * X + Y * Y);
     *J = (1 + X * X) * (1 + Y * Y) / (sg * sg * sg);
-    double Ac[4] = {1, 1, 1, 1};
+    double Ac[6] = {1, 1, 1, 1, 1, 1};
     int nc = 0;
     if (s->order == 42) {
         Ac[0] = 13.0 / 12; Ac[1] = 7.0 / 8; Ac[2] = 25.0 / 24; Ac[3] = 1;
         nc = 4;
+    } else if (s->order == 63) {
+        for (int k = 0; k < 6; k++) Ac[k] = H63_C[k];
+        nc = 6;
     }
     double wx = 1, wy = 1;
     if (i <= nc) wx = Ac[i - 1];

@@ -50,7 +50,7 @@ def _dup_jump(o, f):
     return np.abs(f[hm["tp"], ej, ei] - f[hm["sp"], hm["sj"] - 1, hm["si"] - 1]).max()
 
 
-@pytest.mark.parametrize("order,N", [(21, 12), (21, 17), (42, 12), (42, 20)])
+@pytest.mark.parametrize("order,N", [(21, 12), (21, 17), (42, 12), (42, 20), (63, 14), (63, 24)])
 @pytest.mark.parametrize("cor_const", [False, True])
 def test_energy_neutrality(oracle_mod, order, N, cor_const):
     o = oracle_mod.Oracle(N, order, coriolis_const=cor_const, omega=7.292e-5 if not cor_const else 1e-4)
@@ -58,7 +58,7 @@ def test_energy_neutrality(oracle_mod, order, N, cor_const):
     assert abs(_energy_rate(o, u, v, _smooth_h(o))) < 1e-13
 
 
-@pytest.mark.parametrize("order", [21, 42])
+@pytest.mark.parametrize("order", [21, 42, 63])
 def test_coriolis_neutral_and_mass_conserved(oracle_mod, order):
     o = oracle_mod.Oracle(16, order)
     u, v = _random_uv(o, 3)
@@ -76,7 +76,7 @@ def test_coriolis_neutral_and_mass_conserved(oracle_mod, order):
     assert _dup_jump(o, dh) < 1e-14 * np.abs(dh).max()
 
 
-@pytest.mark.parametrize("order", [21, 42])
+@pytest.mark.parametrize("order", [21, 42, 63])
 def test_co2contra_symmetric_positive(oracle_mod, order):
     o = oracle_mod.Oracle(14, order)
     wu, wv = o.weights(MESH_U), o.weights(MESH_V)
@@ -94,7 +94,7 @@ def _conv(errs):
     return np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
 
 
-@pytest.mark.parametrize("order,rate", [(21, 1.4), (42, 1.8)])
+@pytest.mark.parametrize("order,rate", [(21, 1.4), (42, 1.8), (63, 2.6)])
 def test_grad_convergence(oracle_mod, order, rate):
     """h = c . r on the unit sphere: covariant gradient = c . a / radius."""
     c = np.array([0.3, -0.5, 0.8])
@@ -155,16 +155,17 @@ def _subop_errors(o):
     return e_c2c, e_div, e_cor
 
 
-@pytest.mark.parametrize("order,rate", [(21, 0.75), (42, 1.8)])
+@pytest.mark.parametrize("order,rate", [(21, 0.75), (42, 1.8), (63, 1.95)])
 def test_co2contra_div_coriolis_convergence(oracle_mod, order, rate):
     """Max-norm errors are set by the boundary closures: first order for
-    sbp21, second order for sbp42."""
+    sbp21, second order for sbp42, between second and third for sbp63 (and
+    3 to 10 times smaller than sbp42's at the same N)."""
     errs = np.array([_subop_errors(oracle_mod.Oracle(N, order)) for N in (16, 32, 64)])
     for k in range(3):
         assert np.all(_conv(errs[:, k]) > rate), errs
 
 
-@pytest.mark.parametrize("order", [21, 42])
+@pytest.mark.parametrize("order", [21, 42, 63])
 def test_ts2_linear_steady_state(oracle_mod, order):
     """Geostrophically balanced solid rotation: the tendency is pure
     truncation error and converges to zero (max norm: first order, set by
@@ -185,9 +186,9 @@ def test_ts2_linear_steady_state(oracle_mod, order):
     assert np.all(_conv(el2) > 1.3), el2
 
 
-@pytest.mark.parametrize("order", [21, 42])
+@pytest.mark.parametrize("order", [21, 42, 63])
 def test_rk4_is_taylor_polynomial(oracle_mod, order):
-    o = oracle_mod.Oracle(10, order, coriolis_const=True, omega=1e-4)
+    o = oracle_mod.Oracle(10 if order != 63 else 14, order, coriolis_const=True, omega=1e-4)
     u, v = _random_uv(o, 7)
     h = _smooth_h(o)
     dt = 900.0

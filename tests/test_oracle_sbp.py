@@ -4,14 +4,16 @@ A wrong closure coefficient, a wrong norm weight or a wrong SAT projection
 breaks one of: the summation-by-parts identity, interpolation duality, or
 polynomial exactness.  Coefficients follow
 differential_operators/sbp_operators/sbp_{diff,interp}_{i2c,c2i}_{21,42}_mod.f90
-and sbp_operators_collection_mod.f90 L169-197.
+and sbp_operators_collection_mod.f90 L169-200 (sbp63: sbp_*_63_mod.f90).
 """
 import numpy as np
 import pytest
 
-R_PROJ = {21: [1.5, -0.5], 42: [15 / 8, -5 / 4, 3 / 8]}
-A_INT = {21: [0.5, 1.0], 42: [7 / 18, 9 / 8, 1.0, 71 / 72, 1.0]}
-A_CEN = {21: [1.0, 1.0], 42: [13 / 12, 7 / 8, 25 / 24, 1.0]}
+R_PROJ = {21: [1.5, -0.5], 42: [15 / 8, -5 / 4, 3 / 8], 63: [35 / 16, -35 / 16, 21 / 16, -5 / 16]}
+A_INT = {21: [0.5, 1.0], 42: [7 / 18, 9 / 8, 1.0, 71 / 72, 1.0],
+         63: [95 / 288, 317 / 240, 23 / 30, 793 / 720, 157 / 160]}
+A_CEN = {21: [1.0, 1.0], 42: [13 / 12, 7 / 8, 25 / 24, 1.0],
+         63: [325363 / 276480, 144001 / 276480, 43195 / 27648, 86857 / 138240, 312623 / 276480, 271229 / 276480]}
 
 
 def _mat(m, order, op, n):
@@ -35,9 +37,11 @@ def _weights(order, n):
     return wi, wc
 
 
-@pytest.mark.parametrize("order", [21, 42])
+@pytest.mark.parametrize("order", [21, 42, 63])
 @pytest.mark.parametrize("n", [8, 13, 24])
 def test_summation_by_parts(oracle_mod, order, n):
+    if order == 63 and n < 12:
+        pytest.skip("sbp63 closures need 12 centres")
     m = oracle_mod
     Di2c = _mat(m, order, m.OP_D_I2C, n)
     Dc2i = _mat(m, order, m.OP_D_C2I, n)
@@ -51,9 +55,11 @@ def test_summation_by_parts(oracle_mod, order, n):
     assert np.abs(B - expect).max() < 1e-14
 
 
-@pytest.mark.parametrize("order", [21, 42])
+@pytest.mark.parametrize("order", [21, 42, 63])
 @pytest.mark.parametrize("n", [8, 17])
 def test_interpolation_duality(oracle_mod, order, n):
+    if order == 63 and n < 12:
+        pytest.skip("sbp63 closures need 12 centres")
     m = oracle_mod
     Ii2c = _mat(m, order, m.OP_I_I2C, n)
     Ic2i = _mat(m, order, m.OP_I_C2I, n)
@@ -65,13 +71,16 @@ def test_interpolation_duality(oracle_mod, order, n):
 
 
 @pytest.mark.parametrize("order,interior,boundary,interp_boundary",
-                         [(21, 2, 1, 0), (42, 4, 2, 1)])
+                         [(21, 2, 1, 0), (42, 4, 2, 1), (63, 6, 3, 2)])
 def test_polynomial_exactness(oracle_mod, order, interior, boundary, interp_boundary):
     """Interior rows of the staggered derivatives are exact for polynomials up
-    to degree 2 (sbp21) / 4 (sbp42); closure rows up to degree 1 / 2.  The
-    4th-order boundary interpolation is exact to degree 1 only: its free
-    parameters c13, c14 minimise (not zero) the degree-2 error
-    (sbp_interp_i2c_42_mod.f90 L12-19); the 2nd-order one copies the end value."""
+    to degree 2 (sbp21) / 4 (sbp42) / 6 (sbp63); closure rows up to degree 1
+    / 2 / 3.  The 4th-order boundary interpolation is exact to degree 1 only:
+    its free parameters c13, c14 minimise (not zero) the degree-2 error
+    (sbp_interp_i2c_42_mod.f90 L12-19); the 6th-order one to degree 2 (its
+    six free parameters minimise the degree-3 and 4 errors,
+    sbp_interp_i2c_63_mod.f90 L9-20); the 2nd-order one copies the end
+    value."""
     m = oracle_mod
     n = 24
     xi = np.arange(n + 1, dtype=float)  # interfaces
@@ -80,7 +89,7 @@ def test_polynomial_exactness(oracle_mod, order, interior, boundary, interp_boun
     Dc2i = _mat(m, order, m.OP_D_C2I, n)
     Ii2c = _mat(m, order, m.OP_I_I2C, n)
     Ic2i = _mat(m, order, m.OP_I_C2I, n)
-    lo, hi = (1, n - 1) if order == 21 else (4, n - 4)
+    lo, hi = {21: (1, n - 1), 42: (4, n - 4), 63: (6, n - 6)}[order]
     for deg in range(interior + 1):
         # interior rows exact to degree `interior` for derivatives
         d_i2c = Di2c @ xi**deg
