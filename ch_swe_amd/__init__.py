@@ -1,5 +1,6 @@
 """ch_swe_amd: MI355X-native linear shallow-water RK4 step on the staggered
-(Ch) equiangular cubed sphere, SBP orders 21 and 42.
+(Ch) equiangular cubed sphere, SBP orders 21, 42 and 63 (63 on the
+multi-kernel path).
 
 Python host mirror of the reference's interface for this path:
 
@@ -401,7 +402,10 @@ def create_linear_swm_operator(N, order=42, precision=FP64, grav=9.81, h_mean=10
                                general_tiles=False, compensated=False, tile_x0=0, tile_y0=0, tuned=True):
     """Factory mirroring create_linear_swm_operator
     (operator_swm_factory_mod.f90 L586-685) for the Ch staggering.
-    unfused=True selects the multi-kernel diagnostic path; linear_blocks=True
+    unfused=True selects the multi-kernel diagnostic path (order 63 always
+    runs there: the fused kernels are built for the sbp21 / sbp42 closures, so
+    the diagnostics, ensembles and the server take sbp21 / sbp42 plans only);
+    linear_blocks=True
     the plain block order of the fused stage kernel; general_tiles=True sends
     every tile through its general path; block_threads (0 or 512) its
     workgroup size; compensated=True runs RK4 with compensated summation of

@@ -77,13 +77,16 @@ struct GeomParams {
 
 __device__ inline double sbp_weight(int idx, int n, bool centre, int order)
 {
-    // D21_A_interfaces/centers, D42_A_interfaces/centers
-    // (sbp_operators_collection_mod.f90 L169-170, L196-197)
+    // D21_A_interfaces/centers, D42_A_interfaces/centers, D63_A_interfaces/
+    // centers (sbp_operators_collection_mod.f90 L169-170, L196-200)
     const double i21[2] = {0.5, 1.0}, c21[2] = {1.0, 1.0};
     const double i42[5] = {7.0 / 18, 9.0 / 8, 1.0, 71.0 / 72, 1.0};
     const double c42[4] = {13.0 / 12, 7.0 / 8, 25.0 / 24, 1.0};
-    const double *A = order == 21 ? (centre ? c21 : i21) : (centre ? c42 : i42);
-    int ne = order == 21 ? 2 : (centre ? 4 : 5);
+    const double i63[5] = {95.0 / 288, 317.0 / 240, 23.0 / 30, 793.0 / 720, 157.0 / 160};
+    const double c63[6] = {325363.0 / 276480, 144001.0 / 276480, 43195.0 / 27648,
+                           86857.0 / 138240,  312623.0 / 276480, 271229.0 / 276480};
+    const double *A = order == 21 ? (centre ? c21 : i21) : order == 42 ? (centre ? c42 : i42) : (centre ? c63 : i63);
+    int ne = order == 21 ? 2 : order == 42 ? (centre ? 4 : 5) : (centre ? 6 : 5);
     double w = 1.0;
     if (idx <= ne) w = A[idx - 1];
     if (n - idx + 1 <= ne) w = A[n - idx];

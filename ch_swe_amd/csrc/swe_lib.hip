@@ -65,6 +65,10 @@ constexpr int TPB = 256;
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+// depth of the normal-flux halo the centres -> interfaces derivative's SAT
+// reads: the length of its boundary projection R (2, 3, 4 for sbp21 / 42 / 63)
+inline int ch_depth(int order) { return order == 21 ? 2 : order == 42 ? 3 : 4; }
+
 template <typename T, int O>
 __global__ void k_xy_w(Dims d, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ JQ12,
                        T *__restrict__ W1, T *__restrict__ W2)
@@ -613,7 +617,9 @@ struct Engine : EngineBase {
         geo.order = order;
         k_sync_weights<T><<<blocks_for(d.N + 1), TPB, 0, pl->stream>>>(d.N, geo.hx, qb, qt, ql, qr);
         HIP_TRY(hipGetLastError());
-        use_fused = !(p->cfg.flags & SWE_FLAG_UNFUSED);
+        // sbp63 runs on the multi-kernel path (the fused kernels' regions are
+        // sized for the sbp21 / sbp42 closures)
+        use_fused = !(p->cfg.flags & SWE_FLAG_UNFUSED) && order != 63;
         nthr = p->cfg.block_threads ? p->cfg.block_threads : F_THREADS;
         if ((rc = alloc((void **)&part, 3 * INT_MAX_BLOCKS * sizeof(double))) ||
             (rc = alloc((void **)&red, 4 * sizeof(double))) ||
@@ -645,7 +651,7 @@ struct Engine : EngineBase {
         HIP_TRY(hipGetLastError());
         // maps: field slots 0 and 1 are the (u-type, v-type) components
         if ((rc = upload_map(0, 1, {0, 0}, &maps.xy, &maps.n_xy))) return rc;
-        if ((rc = upload_map(1, order == 21 ? 2 : 3, {0, 1}, &maps.ch, &maps.n_ch))) return rc;
+        if ((rc = upload_map(1, ch_depth(order), {0, 1}, &maps.ch, &maps.n_ch))) return rc;
         if ((rc = upload_map(2, 1, {0, 1}, &maps.xyv, &maps.n_xyv))) return rc;
         HIP_TRY(hipStreamSynchronize(pl->stream));
         full_ready = true;
@@ -863,7 +869,8 @@ struct Engine : EngineBase {
             if (order == 21) fused_apply<21>((T *)out, (const T *)in);
             else fused_apply<42>((T *)out, (const T *)in);
         } else if (order == 21) apply_o<21>((T *)out, (T *)in);
-        else apply_o<42>((T *)out, (T *)in);
+        else if (order == 42) apply_o<42>((T *)out, (T *)in);
+        else apply_o<63>((T *)out, (T *)in);
         HIP_TRY(hipGetLastError());
         return SWE_OK;
     }
@@ -1018,7 +1025,8 @@ struct Engine : EngineBase {
     void xy_dot(const T *a, const T *b, double *out)
     {
         if (order == 21) k_xy_dot<T, 21><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, geo.hx, a, b, sv_part);
-        else k_xy_dot<T, 42><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, geo.hx, a, b, sv_part);
+        else if (order == 42) k_xy_dot<T, 42><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, geo.hx, a, b, sv_part);
+        else k_xy_dot<T, 63><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, geo.hx, a, b, sv_part);
         k_dot_final<<<1, 64, 0, pl->stream>>>(sv_part, DOT_BLOCKS, out);
     }
     // swm_helm_lin_operator_t%apply: az = z - gamma div co2contra grad z with
@@ -1153,9 +1161,13 @@ struct Engine : EngineBase {
                 k_xy_w<T, 21><<<blocks_for(nxy), TPB, 0, s>>>(d, u, v, JQ12, W1, W2);
                 k_contra<T, 21><<<blocks_for(2 * nuv), TPB, 0, s>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut,
                                                                     vt, Jut, Jvt);
-            } else {
+            } else if (order == 42) {
                 k_xy_w<T, 42><<<blocks_for(nxy), TPB, 0, s>>>(d, u, v, JQ12, W1, W2);
                 k_contra<T, 42><<<blocks_for(2 * nuv), TPB, 0, s>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut,
+                                                                    vt, Jut, Jvt);
+            } else {
+                k_xy_w<T, 63><<<blocks_for(nxy), TPB, 0, s>>>(d, u, v, JQ12, W1, W2);
+                k_contra<T, 63><<<blocks_for(2 * nuv), TPB, 0, s>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut,
                                                                     vt, Jut, Jvt);
             }
             k_reduce<T><<<nblk_red, TPB, 0, s>>>(d, h, u, v, ut, vt, wxy, wu, wv, prt);
@@ -1186,9 +1198,15 @@ int validate_config(const swe_config *cfg, const char *who)
 {
     const std::string w = std::string(who) + ": ";
     if (!cfg) return fail(SWE_EINVAL, w + "NULL configuration");
-    if (cfg->order != 21 && cfg->order != 42) return fail(SWE_EINVAL, w + "order must be 21 or 42");
+    if (cfg->order != 21 && cfg->order != 42 && cfg->order != 63)
+        return fail(SWE_EINVAL, w + "order must be 21, 42 or 63");
     if (cfg->order == 21 && cfg->N < 4) return fail(SWE_EINVAL, w + "sbp21 needs N >= 4");
     if (cfg->order == 42 && cfg->N < 8) return fail(SWE_EINVAL, w + "sbp42 needs N >= 8");
+    if (cfg->order == 63 && cfg->N < 12) return fail(SWE_EINVAL, w + "sbp63 needs N >= 12");
+    if (cfg->order == 63 && (cfg->flags & SWE_FLAG_COMPENSATED))
+        return fail(SWE_EINVAL, w + "compensated summation runs on the fused path (sbp21 / sbp42 only)");
+    if (cfg->order == 63 && (cfg->tile_x0 || cfg->tile_y0))
+        return fail(SWE_EINVAL, w + "tile_x0 / tile_y0 apply to the fused path (sbp21 / sbp42 only)");
     if (cfg->N > 4096) return fail(SWE_EINVAL, w + "N above 4096 is not supported");
     if (cfg->precision != SWE_FP64 && cfg->precision != SWE_FP32)
         return fail(SWE_EINVAL, w + "precision must be SWE_FP64 or SWE_FP32");
@@ -1681,9 +1699,9 @@ int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *
                  int *sj, int *sc, int *sign)
 {
     if (!count) return fail(SWE_EINVAL, "NULL count");
-    if (N < 1 || (order != 21 && order != 42) || kind < 0 || kind > 2)
+    if (N < 1 || (order != 21 && order != 42 && order != 63) || kind < 0 || kind > 2)
         return fail(SWE_EINVAL, "swe_halo_map: bad arguments");
-    int depth = kind == 1 ? (order == 21 ? 2 : 3) : 1;
+    int depth = kind == 1 ? ch_depth(order) : 1;
     auto m = swe::build_halo_map(N, kind, depth);
     *count = (int)m.size();
     if (tp) {

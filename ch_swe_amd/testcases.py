@@ -75,8 +75,11 @@ def sbp_weights(N, mesh, order):
     xt, yt = {"u": (1, 0), "v": (0, 1), "h": (0, 0)}[mesh]
     if order == 21:
         ai, ac = [0.5, 1.0], [1.0, 1.0]
-    else:
+    elif order == 42:
         ai, ac = [7 / 18, 9 / 8, 1.0, 71 / 72, 1.0], [13 / 12, 7 / 8, 25 / 24, 1.0]
+    else:
+        ai = [95 / 288, 317 / 240, 23 / 30, 793 / 720, 157 / 160]
+        ac = [325363 / 276480, 144001 / 276480, 43195 / 27648, 86857 / 138240, 312623 / 276480, 271229 / 276480]
 
     def w1(n, centre):
         e = ac if centre else ai

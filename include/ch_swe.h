@@ -56,8 +56,10 @@ typedef struct swe_state swe_state;
  * arguments (grav, h_mean, SBP order via the operator names:
  * models/shallow_water/operator/operator_swm_factory_mod.f90 L586-685). */
 typedef struct {
-    int N;              /* cells per panel edge; >= 4 (sbp21) or >= 8 (sbp42) */
-    int order;          /* 21 or 42 */
+    int N;              /* cells per panel edge; >= 4 (sbp21), >= 8 (sbp42), >= 12 (sbp63) */
+    int order;          /* 21, 42 or 63; 63 runs on the multi-kernel path (as with
+                           SWE_FLAG_UNFUSED): the diagnostics, ensembles and the server
+                           take sbp21 / sbp42 plans */
     int precision;      /* SWE_FP64 or SWE_FP32 (stencils/state); metric is FP64 */
     int coriolis_const; /* 1: f = omega everywhere; 0: f = 2 omega (axis . r) */
     double radius;      /* planet radius [m] */

@@ -14,7 +14,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define HALO 4 /* widest read across an edge: c2i SAT depth 3 */
+#define HALO 4 /* widest read across an edge: the sbp63 c2i SAT, depth 4 */
 
 /* Row loops are split over OpenMP threads when built with -fopenmp (used for
  * the timed CPU baseline).  Every parallel loop writes disjoint points and

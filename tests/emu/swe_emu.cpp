@@ -22,6 +22,7 @@
 //
 // usage: swe_emu <case> N order prec [steps] [seed] [flags]
 //   case  : apply | rk4 | integrals | diag | tends | latlon | latlonuv | multi
+//           | lines (the 1-D point stencils; the only case for order 63)
 //   prec  : 64 | 32
 //   flags : bit 0 every tile on the general path, bit 1 plain block order,
 //           bit 2 compensated summation (rk4)
@@ -813,9 +814,58 @@ int run_multi(int N, int N2, int steps, unsigned seed)
     return bad;
 }
 
+// The 1-D SBP stencils of swe_sbp.hpp (the multi-kernel path's point
+// functions, SAT terms included) on random lines of n = N .. N + 3 centres with
+// random halo values, against the oracle's line operators.
+template <typename T, int O>
+int run_lines(int N, unsigned seed)
+{
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    const double bar = sizeof(T) == 8 ? 1e-13 : 2e-6;
+    int bad = 0;
+    for (int n = N; n < N + 4; n++) {
+        const int len = n + 1 + 2 * 4;  // index 1 - 4 .. n + 1 + 4
+        std::vector<double> in(len), ref(n + 1);
+        std::vector<T> tin(len);
+        for (int k = 0; k < len; k++) {
+            in[k] = U(rng);
+            tin[k] = T(in[k]);
+            in[k] = double(tin[k]);
+        }
+        const T *f0 = tin.data() + 4 - 1;  // index 0
+        static const char *names[4] = {"d_i2c", "d_c2i", "i_i2c", "i_c2i"};
+        for (int op = 0; op < 4; op++) {
+            swo_line_op(O, op, n, in.data(), ref.data(), 1);
+            const int nout = (op == 0 || op == 2) ? n : n + 1;
+            double err = 0, mag = 0;
+            for (int k = 1; k <= nout; k++) {
+                auto f = swe::line(f0, 1);
+                T g = op == 0   ? swe::d_i2c_pt<O, T>(f, 0, n, k)
+                      : op == 1 ? swe::d_c2i_pt<O, T>(f, 0, n, k)
+                      : op == 2 ? swe::i_i2c_pt<O, T>(f, 0, n, k)
+                                : swe::i_c2i_pt<O, T>(f, 0, n, k);
+                err = std::max(err, std::fabs(double(g) - ref[k - 1]));
+                mag = std::max(mag, std::fabs(ref[k - 1]));
+            }
+            const double rel = err / std::max(mag, 1e-300);
+            std::printf("  %-6s sbp%d n=%d rel %.3e%s\n", names[op], O, n, rel, rel > bar ? "  <-- FAIL" : "");
+            bad |= rel > bar;
+        }
+    }
+    return bad;
+}
+
 template <typename T>
 int dispatch(const std::string &cs, int N, int order, int steps, unsigned seed, int flags)
 {
+    if (cs == "lines")
+        return order == 21 ? run_lines<T, 21>(N, seed) : order == 42 ? run_lines<T, 42>(N, seed)
+                                                                    : run_lines<T, 63>(N, seed);
+    if (order == 63) {
+        std::fprintf(stderr, "sbp63 runs on the multi-kernel path: only the lines case applies\n");
+        return 1;
+    }
     if (cs == "multi") {
         // the second member: a different N with the same order
         const int N2 = N + 7;
@@ -829,7 +879,7 @@ int dispatch(const std::string &cs, int N, int order, int steps, unsigned seed, 
 int main(int argc, char **argv)
 {
     if (argc < 5) {
-        std::fprintf(stderr, "usage: %s apply|rk4|integrals|diag|tends|latlon|latlonuv|multi N order 64|32 [steps] [seed] [flags]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s apply|rk4|integrals|diag|tends|latlon|latlonuv|multi|lines N order 64|32 [steps] [seed] [flags]\n", argv[0]);
         return 2;
     }
     const std::string cs = argv[1];
@@ -837,7 +887,8 @@ int main(int argc, char **argv)
     const int steps = argc > 5 ? std::atoi(argv[5]) : 1;
     const unsigned seed = argc > 6 ? (unsigned)std::atoi(argv[6]) : 1u;
     const int flags = argc > 7 ? std::atoi(argv[7]) : 0;
-    if ((order != 21 && order != 42) || (prec != 64 && prec != 32) || N < (order == 21 ? 4 : 8)) {
+    if ((order != 21 && order != 42 && order != 63) || (prec != 64 && prec != 32) ||
+        N < (order == 21 ? 4 : order == 42 ? 8 : 12)) {
         std::fprintf(stderr, "bad arguments\n");
         return 2;
     }

@@ -30,8 +30,12 @@ def test_plan_rejects_bad_sizes(N, order):
 
 
 def test_plan_rejects_bad_order():
-    with pytest.raises(cs.SweError, match="order"):
-        cs.create_linear_swm_operator(16, 63)
+    with pytest.raises(cs.SweError, match="order must be 21, 42 or 63"):
+        cs.create_linear_swm_operator(16, 84)
+    with pytest.raises(cs.SweError, match="sbp63 needs N >= 12"):
+        cs.create_linear_swm_operator(11, 63)
+    with pytest.raises(cs.SweError, match="compensated"):
+        cs.create_linear_swm_operator(16, 63, compensated=True)
 
 
 def _entries(m):
@@ -40,11 +44,11 @@ def _entries(m):
 
 
 @pytest.mark.parametrize("N", [4, 9, 16])
-@pytest.mark.parametrize("order", [21, 42])
+@pytest.mark.parametrize("order", [21, 42, 63])
 @pytest.mark.parametrize("kind", [0, 1, 2])
 def test_edge_maps_match_oracle(oracle_mod, N, order, kind):
-    if order == 42 and N < 8:
-        pytest.skip("sbp42 needs N >= 8")
+    if N < {21: 4, 42: 8, 63: 12}[order]:
+        pytest.skip(f"sbp{order} needs a larger N")
     o = oracle_mod.Oracle(N, order)
     assert _entries(cs.halo_map(N, order, kind)) == _entries(o.halo_map(kind))
 

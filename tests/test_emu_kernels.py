@@ -45,6 +45,10 @@ CASES = [
     ("latlon", 17, 21, 32, 1, 23, 0),
     ("latlonuv", 12, 42, 64, 1, 24, 0),  # lat-lon winds (colocation, vector halo map, zonal / meridional)
     ("latlonuv", 17, 21, 32, 1, 25, 0),
+    ("lines", 12, 63, 64, 1, 26, 0),     # the multi-kernel path's 1-D stencils, SAT included
+    ("lines", 13, 63, 32, 1, 27, 0),
+    ("lines", 8, 42, 64, 1, 28, 0),
+    ("lines", 4, 21, 64, 1, 29, 0),
 ]
 
 
