@@ -404,7 +404,7 @@ def create_linear_swm_operator(N, order=42, precision=FP64, grav=9.81, h_mean=10
     (operator_swm_factory_mod.f90 L586-685) for the Ch staggering.
     unfused=True selects the multi-kernel diagnostic path (order 63 always
     runs there: the fused kernels are built for the sbp21 / sbp42 closures, so
-    the diagnostics, ensembles and the server take sbp21 / sbp42 plans only);
+    ensembles and the server take sbp21 / sbp42 plans only);
     linear_blocks=True
     the plain block order of the fused stage kernel; general_tiles=True sends
     every tile through its general path; block_threads (0 or 512) its

@@ -390,6 +390,180 @@ __global__ void k_reduce_final(const double *part, int nb, double *out)
 #include "swe_fused_host.hpp"
 #include "swe_latlon.inc"
 
+// ---- diagnostics on the multi-kernel path (any SBP order) -------------------
+// The same sums as k_diag / k_tends (print_integrals_diag, print_CFL_diag and
+// print_integral_tends: swm_output_diag_mod.f90 L110-342), point by point in
+// FP64 from the plan's fields: the interpolations to the u / v points and the
+// centres are evaluated inside each point's sum (h -> hu -> hq as nested line
+// stencils), the curl reads copies of u, v whose tangential duplicates were
+// gathered from the neighbours.  Blocks write partials in k_diag's and
+// k_tends' layouts, combined by the same final kernels.
+constexpr int MK_BLOCKS = 1024;
+
+// cell-centre (mesh o) J and f, FP64 (metric_2d_ecs_mod.f90 L338-411 at the o
+// points; coriolis_factory_mod.f90 calc_coriolis_parameter L239-280)
+__device__ inline void mk_centre(const GeomParams &g, int p, int i, int j, double &J, double &f)
+{
+    const double X = tan(-0.25 * PI + (i - 0.5) * g.hx), Y = tan(-0.25 * PI + (j - 0.5) * g.hx);
+    const double s2 = 1.0 + X * X + Y * Y, s = sqrt(s2);
+    J = (1.0 + X * X) * (1.0 + Y * Y) / (s2 * s);
+    if (g.coriolis_const) {
+        f = g.omega;
+    } else {
+        double r[3];
+        for (int k = 0; k < 3; k++) r[k] = (c_EX[p][k] * X + c_EY[p][k] * Y - c_NN[p][k]) / s;
+        const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        f = 2.0 * g.omega * (g.axis[0] * r[0] + g.axis[1] * r[1] + g.axis[2] * r[2]) / rn;
+    }
+}
+
+// h at u point (i, j): i2c along x of xy row j; at v point (i, j): along y
+template <int O, typename T>
+__device__ inline double mk_hu(const Dims &d, const T *h, int p, int i, int j)
+{
+    return swe::i_i2c_pt<O, double>([&](int k) { return double(h[off(d, p, k, j)]); }, 0, d.N, i);
+}
+template <int O, typename T>
+__device__ inline double mk_hv(const Dims &d, const T *h, int p, int i, int j)
+{
+    return swe::i_i2c_pt<O, double>([&](int k) { return double(h[off(d, p, i, k)]); }, 0, d.N, j);
+}
+// at centre (i, j): i2c along y of hu (the first output of interp2d_uv2qvec_Ch)
+// and i2c along x of hv (its second)
+template <int O, typename T>
+__device__ inline double mk_hq1(const Dims &d, const T *h, int p, int i, int j)
+{
+    return swe::i_i2c_pt<O, double>([&](int k) { return mk_hu<O>(d, h, p, i, k); }, 0, d.N, j);
+}
+template <int O, typename T>
+__device__ inline double mk_hq2(const Dims &d, const T *h, int p, int i, int j)
+{
+    return swe::i_i2c_pt<O, double>([&](int k) { return mk_hv<O>(d, h, p, k, j); }, 0, d.N, i);
+}
+// curl_ch_sbp_sat at centre (i, j) (curl_sbp_sat_mod.f90 L26-74): i2c
+// derivatives of v along x and u along y, SAT against the tangential
+// duplicates held in ua / va at indices 0 and N + 2; J from mk_centre
+template <int O, typename T>
+__device__ inline double mk_curl(const Dims &d, const GeomParams &g, const T *ua, const T *va, int p, int i, int j,
+                                 double J)
+{
+    const double Dx = swe::d_i2c_pt<O, double>([&](int k) { return double(va[off(d, p, k, j)]); }, 0, d.N, i);
+    const double Dy = swe::d_i2c_pt<O, double>([&](int k) { return double(ua[off(d, p, i, k)]); }, 0, d.N, j);
+    return (Dx / g.hx - Dy / g.hx) / (J * g.radius);
+}
+
+template <int K>
+__device__ inline void mk_block_partials(double (&s)[K], const int (&kind)[K], double *part)
+{
+    // kind 0: sum, 1: min, 2: max
+    __shared__ double ws[K][4];
+    for (int q = 0; q < K; q++) {
+        double x = s[q];
+        for (int m = 32; m >= 1; m >>= 1) {
+            const double y = __shfl_down(x, m);
+            x = kind[q] == 0 ? x + y : kind[q] == 1 ? fmin(x, y) : fmax(x, y);
+        }
+        if ((threadIdx.x & 63) == 0) ws[q][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        const int q = threadIdx.x;
+        double x = ws[q][0];
+        for (int w = 1; w < 4; w++) x = kind[q] == 0 ? x + ws[q][w] : kind[q] == 1 ? fmin(x, ws[q][w]) : fmax(x, ws[q][w]);
+        part[K * blockIdx.x + q] = x;
+    }
+}
+
+// hmin, hmax, mass, KE, PE, enstrophy, max |c ut u|, max |c vt v| over the
+// xy, u, v points and the centres (one grid-stride index space)
+template <typename T, int O>
+__global__ __launch_bounds__(256) void k_mk_diag(Dims d, GeomParams g, const T *u, const T *v, const T *h,
+                                                 const T *ut, const T *vt, const T *ua, const T *va, double grav,
+                                                 double cfl_c, double *part)
+{
+    const int N = d.N;
+    const long nxy = 6L * (N + 1) * (N + 1), nuv = 6L * N * (N + 1), no = 6L * N * N;
+    double s[DIAG_K] = {INFINITY, -INFINITY, 0, 0, 0, 0, 0, 0};
+    for (long n = blockIdx.x * 256L + threadIdx.x; n < nxy + 2 * nuv + no; n += gridDim.x * 256L) {
+        int p, i, j;
+        if (n < nxy) {
+            unflat(n, N + 1, N + 1, p, i, j);
+            const double hv = h[off(d, p, i, j)], w = quad_weight(N, O, g.hx, 0, i, j);
+            s[0] = fmin(s[0], hv);
+            s[1] = fmax(s[1], hv);
+            s[2] += hv * w;
+            s[4] += 0.5 * grav * hv * hv * w;
+        } else if (n < nxy + nuv) {
+            unflat(n - nxy, N, N + 1, p, i, j);
+            const long o = off(d, p, i, j);
+            const double uu = u[o], uc = ut[o];
+            s[3] += 0.5 * (mk_hu<O>(d, h, p, i, j) * uc) * uu * quad_weight(N, O, g.hx, 1, i, j);
+            s[6] = fmax(s[6], fabs(cfl_c * uc * uu));
+        } else if (n < nxy + 2 * nuv) {
+            unflat(n - nxy - nuv, N + 1, N, p, i, j);
+            const long o = off(d, p, i, j);
+            const double vv = v[o], vc = vt[o];
+            s[3] += 0.5 * (mk_hv<O>(d, h, p, i, j) * vc) * vv * quad_weight(N, O, g.hx, 2, i, j);
+            s[7] = fmax(s[7], fabs(cfl_c * vc * vv));
+        } else {
+            unflat(n - nxy - 2 * nuv, N, N, p, i, j);
+            double J, f;
+            mk_centre(g, p, i, j, J, f);
+            const double hq = 0.5 * mk_hq1<O>(d, h, p, i, j) + 0.5 * mk_hq2<O>(d, h, p, i, j);
+            const double c = mk_curl<O>(d, g, ua, va, p, i, j, J) + f;
+            s[5] += (0.5 * c * c) / hq * quad_weight(N, O, g.hx, 3, i, j);
+        }
+    }
+    constexpr int kind[DIAG_K] = {1, 2, 0, 0, 0, 0, 2, 2};
+    mk_block_partials<DIAG_K>(s, kind, part);
+}
+
+// ke1, ke2, pe, e1, e2 of print_integral_tends: state (u, v, h) with
+// contravariant uc, vc and tangential copies ua, va; tendency (tu, tv, th)
+// with tangential copies tua, tva
+template <typename T, int O>
+__global__ __launch_bounds__(256) void k_mk_tends(Dims d, GeomParams g, const T *u, const T *v, const T *h,
+                                                  const T *uc, const T *vc, const T *ua, const T *va, const T *tu,
+                                                  const T *tv, const T *th, const T *tua, const T *tva, double grav,
+                                                  double *part)
+{
+    const int N = d.N;
+    const long nxy = 6L * (N + 1) * (N + 1), nuv = 6L * N * (N + 1), no = 6L * N * N;
+    double s[TENDS_K] = {0, 0, 0, 0, 0};
+    for (long n = blockIdx.x * 256L + threadIdx.x; n < nxy + 2 * nuv + no; n += gridDim.x * 256L) {
+        int p, i, j;
+        if (n < nxy) {
+            unflat(n, N + 1, N + 1, p, i, j);
+            const long o = off(d, p, i, j);
+            s[2] += grav * double(h[o]) * double(th[o]) * quad_weight(N, O, g.hx, 0, i, j);
+        } else if (n < nxy + nuv) {
+            unflat(n - nxy, N, N + 1, p, i, j);
+            const long o = off(d, p, i, j);
+            const double w = quad_weight(N, O, g.hx, 1, i, j), c = uc[o];
+            s[0] += 0.5 * (mk_hu<O>(d, th, p, i, j) * c) * double(u[o]) * w;
+            s[1] += (mk_hu<O>(d, h, p, i, j) * c) * double(tu[o]) * w;
+        } else if (n < nxy + 2 * nuv) {
+            unflat(n - nxy - nuv, N + 1, N, p, i, j);
+            const long o = off(d, p, i, j);
+            const double w = quad_weight(N, O, g.hx, 2, i, j), c = vc[o];
+            s[0] += 0.5 * (mk_hv<O>(d, th, p, i, j) * c) * double(v[o]) * w;
+            s[1] += (mk_hv<O>(d, h, p, i, j) * c) * double(tv[o]) * w;
+        } else {
+            unflat(n - nxy - 2 * nuv, N, N, p, i, j);
+            double J, f;
+            mk_centre(g, p, i, j, J, f);
+            const double w = quad_weight(N, O, g.hx, 3, i, j);
+            const double hq = mk_hq1<O>(d, h, p, i, j), hqt = mk_hq1<O>(d, th, p, i, j);
+            const double c = mk_curl<O>(d, g, ua, va, p, i, j, J) + f;
+            const double zt = mk_curl<O>(d, g, tua, tva, p, i, j, J);
+            s[3] += (c * zt) / hq * w;
+            s[4] += (((-0.5 * c * c) * hqt) / hq) / hq * w;
+        }
+    }
+    constexpr int kind[TENDS_K] = {0, 0, 0, 0, 0};
+    mk_block_partials<TENDS_K>(s, kind, part);
+}
+
 // ------------------------------------------------------------------------
 // engine
 // ------------------------------------------------------------------------
@@ -490,8 +664,8 @@ struct EngineBase {
 };
 
 struct Maps {
-    GatherEntry *xy = nullptr, *ch = nullptr, *xyv = nullptr;
-    int n_xy = 0, n_ch = 0, n_xyv = 0;
+    GatherEntry *xy = nullptr, *ch = nullptr, *xyv = nullptr, *tan = nullptr;
+    int n_xy = 0, n_ch = 0, n_xyv = 0, n_tan = 0;
 };
 
 }  // namespace
@@ -653,6 +827,7 @@ struct Engine : EngineBase {
         if ((rc = upload_map(0, 1, {0, 0}, &maps.xy, &maps.n_xy))) return rc;
         if ((rc = upload_map(1, ch_depth(order), {0, 1}, &maps.ch, &maps.n_ch))) return rc;
         if ((rc = upload_map(2, 1, {0, 1}, &maps.xyv, &maps.n_xyv))) return rc;
+        if ((rc = upload_map(3, 1, {0, 1}, &maps.tan, &maps.n_tan))) return rc;
         HIP_TRY(hipStreamSynchronize(pl->stream));
         full_ready = true;
         return SWE_OK;
@@ -894,22 +1069,58 @@ struct Engine : EngineBase {
                                                               dpart);
         k_diag_final<<<1, 64, 0, pl->stream>>>(dpart, grid, dout);
     }
+    // u, v of a state copied into (ua, va) with their tangential duplicates
+    // (the curl's SAT values) gathered from the neighbours
+    void tan_copies(const T *st, T *ua, T *va)
+    {
+        k_axpy<T><<<blocks_for(d.F), TPB, 0, pl->stream>>>(d.F, ua, T(1), st, T(0), nullptr, 0);
+        k_axpy<T><<<blocks_for(d.F), TPB, 0, pl->stream>>>(d.F, va, T(1), st + d.F, T(0), nullptr, 0);
+        gather(maps.tan, maps.n_tan, ua, va);
+    }
+    // the multi-kernel path's contravariant components of a state into ut, vt
+    template <int O>
+    void contra_o(const T *u, const T *v)
+    {
+        const long nxy = 6L * (d.N + 1) * (d.N + 1), nuv = 6L * d.N * (d.N + 1);
+        k_xy_w<T, O><<<blocks_for(nxy), TPB, 0, pl->stream>>>(d, u, v, JQ12, W1, W2);
+        k_contra<T, O><<<blocks_for(2 * nuv), TPB, 0, pl->stream>>>(d, u, v, W1, W2, Qu, iJu, Ju, Qv, iJv, Jv, ut, vt,
+                                                                     Jut, Jvt);
+    }
+    template <int O>
+    void mk_diag(const T *st, double grav)
+    {
+        const double dx = geo.radius * geo.hx;
+        contra_o<O>(st, st + d.F);
+        tan_copies(st, up, vp);
+        k_mk_diag<T, O><<<MK_BLOCKS, 256, 0, pl->stream>>>(d, geo, st, st + d.F, st + 2 * d.F, ut, vt, up, vp, grav,
+                                                            1.0 / (dx * dx), dpart);
+        k_diag_final<<<1, 64, 0, pl->stream>>>(dpart, MK_BLOCKS, dout);
+    }
+    // the diagnostics' sums of a state into dout (fused kernels for sbp21 /
+    // sbp42 fused plans, the multi-kernel ones otherwise)
+    int run_diag(const T *st, double grav)
+    {
+        if (use_fused) {
+            if (order == 21) launch_diag<21>(st, grav);
+            else launch_diag<42>(st, grav);
+        } else if (order == 21) mk_diag<21>(st, grav);
+        else if (order == 42) mk_diag<42>(st, grav);
+        else mk_diag<63>(st, grav);
+        HIP_TRY(hipGetLastError());
+        return SWE_OK;
+    }
     int integrals_diag(void *st, double grav, double out[7]) override
     {
-        if (!use_fused) return fail(SWE_EINVAL, "swe_integrals_diag: diagnostics run on fused plans only");
-        if (order == 21) launch_diag<21>((const T *)st, grav);
-        else launch_diag<42>((const T *)st, grav);
-        HIP_TRY(hipGetLastError());
+        int rc = run_diag((const T *)st, grav);
+        if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(out, dout, 7 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
         HIP_TRY(hipStreamSynchronize(pl->stream));
         return SWE_OK;
     }
     int cfl_diag(void *st, double dt, double grav, double out[3]) override
     {
-        if (!use_fused) return fail(SWE_EINVAL, "swe_cfl_diag: diagnostics run on fused plans only");
-        if (order == 21) launch_diag<21>((const T *)st, grav);
-        else launch_diag<42>((const T *)st, grav);
-        HIP_TRY(hipGetLastError());
+        int rc = run_diag((const T *)st, grav);
+        if (rc) return rc;
         double v[9];
         HIP_TRY(hipMemcpyAsync(v, dout, 9 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
         HIP_TRY(hipStreamSynchronize(pl->stream));
@@ -1119,13 +1330,27 @@ struct Engine : EngineBase {
         k_tends<T, O, TY><<<grid, F_THREADS, 0, pl->stream>>>(d, cf, geo, st, tbuf, ntx, nty, fwx, fwy, grav, dpart);
         k_tends_final<<<1, 64, 0, pl->stream>>>(dpart, grid, dout);
     }
+    template <int O>
+    void mk_tends(const T *st, double grav)
+    {
+        const T *t = tbuf;
+        contra_o<O>(st, st + d.F);  // ut, vt; W1, W2 are free after it
+        tan_copies(st, up, vp);
+        tan_copies(t, W1, W2);
+        k_mk_tends<T, O><<<MK_BLOCKS, 256, 0, pl->stream>>>(d, geo, st, st + d.F, st + 2 * d.F, ut, vt, up, vp, t,
+                                                             t + d.F, t + 2 * d.F, W1, W2, grav, dpart);
+        k_tends_final<<<1, 64, 0, pl->stream>>>(dpart, MK_BLOCKS, dout);
+    }
     int integral_tends(void *st, double grav, double out[2]) override
     {
-        if (!use_fused) return fail(SWE_EINVAL, "swe_integral_tends: diagnostics run on fused plans only");
         int rc;
         if (!tbuf && (rc = alloc((void **)&tbuf, 3 * (size_t)d.F * sizeof(T)))) return rc;
         if ((rc = apply(tbuf, st))) return rc;
-        if (order == 21) launch_tends<21>((const T *)st, grav);
+        if (!use_fused) {
+            if (order == 21) mk_tends<21>((const T *)st, grav);
+            else if (order == 42) mk_tends<42>((const T *)st, grav);
+            else mk_tends<63>((const T *)st, grav);
+        } else if (order == 21) launch_tends<21>((const T *)st, grav);
         else launch_tends<42>((const T *)st, grav);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(out, dout, 2 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
@@ -1699,7 +1924,7 @@ int swe_halo_map(int N, int order, int kind, int *count, int *tp, int *ti, int *
                  int *sj, int *sc, int *sign)
 {
     if (!count) return fail(SWE_EINVAL, "NULL count");
-    if (N < 1 || (order != 21 && order != 42 && order != 63) || kind < 0 || kind > 2)
+    if (N < 1 || (order != 21 && order != 42 && order != 63) || kind < 0 || kind > 3)
         return fail(SWE_EINVAL, "swe_halo_map: bad arguments");
     int depth = kind == 1 ? ch_depth(order) : 1;
     auto m = swe::build_halo_map(N, kind, depth);

@@ -108,7 +108,14 @@ std::vector<HaloEntry> build_halo_map(int N, int kind, int depth)
     for (int p = 0; p < 6; p++)
         for (int side = 0; side < 4; side++) {
             bool lr = side == LEFT || side == RIGHT;
-            if (kind == 1) {
+            if (kind == 3) {
+                // tangential components at the duplicated edge points: v (mesh
+                // x) across left/right, u (mesh y) across bottom/top, depth 1
+                // (the neighbour's copy of the edge point): the f(0), f(n + 2)
+                // that the i2c SAT of the curl reads (curl_sbp_sat_mod.f90
+                // L26-44 with sbp_diff_i2c_*_mod.f90 add_SAT_correction)
+                for (int t = 1; t <= N; t++) out.push_back(halo_point(N, p, side, 1, t, lr ? MESH_V : MESH_U, lr ? 1 : 0));
+            } else if (kind == 1) {
                 // Ch vector: u (mesh y) across left/right, v (mesh x) across
                 // bottom/top; the across axis is the centre-type one
                 MeshType mt = lr ? MESH_U : MESH_V;

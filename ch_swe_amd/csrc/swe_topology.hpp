@@ -45,7 +45,8 @@ struct HaloEntry {
 };
 
 // kind 0: xy scalar depth 1; 1: Ch vector (u left/right, v bottom/top) of
-// the given depth; 2: xy vector depth 1.
+// the given depth; 2: xy vector depth 1; 3: the tangential components at the
+// duplicated edge points (v left/right, u bottom/top), depth 1.
 std::vector<HaloEntry> build_halo_map(int N, int kind, int depth);
 
 }  // namespace swe

@@ -16,7 +16,8 @@
 !            'div_ch_sbp_sat_XX_proj', grad_op_name 'grad_ch_sbp_sat_XX',
 !            co2contra_op_name 'co2contra_ch_sbpXX', coriolis_op_name
 !            'coriolis_Ch_sbpXX', quadrature_name 'SBP_CXX_quadrature' with
-!            one XX in {21, 42} (operator_swm_factory_mod.f90 L586-685);
+!            one XX in {21, 42, 63} (operator_swm_factory_mod.f90 L586-685;
+!            sbp63 plans run on the multi-kernel path);
 !   timescheme_name 'rk4', 'implicit_Eul' or 'crank_nikolson' (the last two
 !            with the operator section's helm_solver_cfg: solver_name 'cg',
 !            max_iter, rel_tol, verbose; iterative_solver_factory_mod.f90
@@ -322,7 +323,7 @@ contains
         if (val /= expected) call swe_hip_abort(key//" = '"//val//"': only '"//expected//"' is supported")
     end subroutine expect_str
 
-    ! the SBP order XX of an operator name prefixXXsuffix (21 or 42)
+    ! the SBP order XX of an operator name prefixXXsuffix (21, 42 or 63)
     integer function sbp_order(key, prefix, suffix)
         character(len=*), intent(in) :: key, prefix, suffix
         character(len=:), allocatable :: val
@@ -330,8 +331,10 @@ contains
         sbp_order = 0
         if (val == prefix//"21"//suffix) sbp_order = 21
         if (val == prefix//"42"//suffix) sbp_order = 42
+        if (val == prefix//"63"//suffix) sbp_order = 63
         if (sbp_order == 0) call swe_hip_abort(key//" = '"//val//"': supported are "// &
-                                               prefix//"21"//suffix//" and "//prefix//"42"//suffix)
+                                               prefix//"21"//suffix//", "//prefix//"42"//suffix//" and "// &
+                                               prefix//"63"//suffix)
     end function sbp_order
 
     subroutine same_order(o, key)

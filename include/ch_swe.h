@@ -58,8 +58,8 @@ typedef struct swe_state swe_state;
 typedef struct {
     int N;              /* cells per panel edge; >= 4 (sbp21), >= 8 (sbp42), >= 12 (sbp63) */
     int order;          /* 21, 42 or 63; 63 runs on the multi-kernel path (as with
-                           SWE_FLAG_UNFUSED): the diagnostics, ensembles and the server
-                           take sbp21 / sbp42 plans */
+                           SWE_FLAG_UNFUSED): ensembles and the server take sbp21 /
+                           sbp42 plans */
     int precision;      /* SWE_FP64 or SWE_FP32 (stencils/state); metric is FP64 */
     int coriolis_const; /* 1: f = omega everywhere; 0: f = 2 omega (axis . r) */
     double radius;      /* planet radius [m] */
@@ -181,8 +181,8 @@ int swe_crank_nicolson_steps(swe_plan *plan, swe_state *v0, double dt, int nstep
  * covariant winds: out[3] = the gravity-wave CFL sqrt(max g h / dx^2) dt
  * and the advective CFLs sqrt(max |ut u| / dx^2) dt, sqrt(max |vt v| /
  * dy^2) dt (ut, vt contravariant; dx = dy = radius * pi / (2N)); grav as the
- * reference passes it there (Earth_grav).  Fused plans only; blocks until
- * the values are on the host. */
+ * reference passes it there (Earth_grav).  Blocks until the values are on
+ * the host. */
 int swe_cfl_diag(swe_plan *plan, swe_state *st, double dt, double grav, double out[3]);
 
 /* The lat-lon output of h: swm_output_diag_t%write_fields's h.dat
@@ -267,8 +267,9 @@ int swe_plan_kernel_times(swe_plan *plan, double *stage_ms, int *n_stage, double
 
 /* Host-only (no device needed): halo map of the in-device edge gathers, for
  * inspection and tests.  kind 0: xy scalar depth 1; 1: Ch vector (u across
- * left/right, v across bottom/top) depth 2 (sbp21) or 3 (sbp42); 2: xy
- * vector depth 1.  Same entry convention as the exchange builders of
+ * left/right, v across bottom/top) depth 2, 3 or 4 (sbp21, 42, 63); 2: xy
+ * vector depth 1; 3: the tangential components at the duplicated edge points
+ * (v across left/right, u across bottom/top) depth 1, read by the curl.  Same entry convention as the exchange builders of
  * parallel/exchange_factory_mod.f90 L166-672: target (panel 0..5, i, j,
  * component) <- sign * source (panel, i, j, component).  Pass NULL arrays to
  * query the count. */

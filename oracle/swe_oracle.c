@@ -1838,7 +1838,7 @@ void swo_ts2_linear(const swo *s, double u0, double h_mean, double *u,
 int swo_halo_map(const swo *s, int kind, int *tp, int *ti, int *tj, int *tc,
                  int *sp, int *si, int *sj, int *sc, int *sign)
 {
-    const hmap *m = kind == 0 ? &s->m_xy : (kind == 1 ? &s->m_ch : &s->m_xyv);
+    const hmap *m = kind == 0 ? &s->m_xy : kind == 1 ? &s->m_ch : kind == 3 ? &s->m_tan : &s->m_xyv;
     if (tp)
         for (int k = 0; k < m->n; k++) {
             const hentry *e = &m->e[k];

@@ -92,7 +92,8 @@ void swo_covariant_from_cart(const swo *s, const double *vx, const double *vy,
                              const double *vz, int mesh, double *out);
 
 /* halo maps: kind 0 = xy scalar depth 1, kind 1 = Ch vector (u x-sides,
- * v y-sides) depth 1..3.  Each entry: target (panel, i, j, comp) and source
+ * v y-sides) depth 1..4, kind 2 = xy vector depth 1, kind 3 = tangential
+ * components at the duplicated edge points (the curl's SAT values).  Each entry: target (panel, i, j, comp) and source
  * (panel, i, j, comp), sign.  comp: 0 = u-type, 1 = v-type.  Returns count;
  * pass NULL arrays to query the count. */
 int swo_halo_map(const swo *s, int kind, int *tp, int *ti, int *tj, int *tc,

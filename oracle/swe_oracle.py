@@ -292,7 +292,8 @@ class Oracle:
         return a1, a2
 
     def halo_map(self, kind):
-        """kind 0: xy scalar depth 1; 1: Ch vector; 2: xy vector depth 1."""
+        """kind 0: xy scalar depth 1; 1: Ch vector; 2: xy vector depth 1;
+        3: tangential components at the duplicated edge points."""
         n = self.lib.swo_halo_map(self._h, kind, *([None] * 9))
         arrs = [np.zeros(n, dtype=np.int32) for _ in range(9)]
         self.lib.swo_halo_map(self._h, kind, *[a.ctypes.data_as(_ip) for a in arrs])

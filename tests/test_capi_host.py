@@ -45,7 +45,7 @@ def _entries(m):
 
 @pytest.mark.parametrize("N", [4, 9, 16])
 @pytest.mark.parametrize("order", [21, 42, 63])
-@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
 def test_edge_maps_match_oracle(oracle_mod, N, order, kind):
     if N < {21: 4, 42: 8, 63: 12}[order]:
         pytest.skip(f"sbp{order} needs a larger N")

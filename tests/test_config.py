@@ -230,7 +230,7 @@ def _run_model(tmp_path, text, *args):
 
 @pytest.mark.parametrize("edit,match", [
     (("'Ch'", "'Ah'"), "horizontal_staggering = 'Ah'"),
-    (("div_ch_sbp_sat_21_proj'", "div_ch_sbp_sat_63_proj'"), "div_op_name = 'div_ch_sbp_sat_63_proj'"),
+    (("div_ch_sbp_sat_21_proj'", "div_ch_sbp_sat_84_proj'"), "div_op_name = 'div_ch_sbp_sat_84_proj'"),
     (("grad_ch_sbp_sat_21'", "grad_ch_sbp_sat_42'"), "grad_op_name: SBP order differs"),
     (("'linear_swm_operator'", "'vector_invariant_swm_operator'"), "swm_operator_type"),
     (("'covariant'", "'contravariant'"), "v_components_type = 'contravariant'"),

@@ -42,7 +42,7 @@ def _rel(g, r):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("order,precision,tol", [(21, "fp64", 1e-12), (42, "fp64", 1e-12),
+@pytest.mark.parametrize("order,precision,tol", [(21, "fp64", 1e-12), (42, "fp64", 1e-12), (63, "fp64", 1e-12),
                                                  (21, "fp32", 5e-5), (42, "fp64-compensated", 1e-12)])
 def test_gauss_config_matches_oracle(oracle_mod, tmp_path, order, precision, tol):
     """linear_gauss_run.sh's corner case, N=32, dt=900 s, 2 hours (8 steps),
@@ -101,7 +101,7 @@ def test_gauss_config_matches_oracle(oracle_mod, tmp_path, order, precision, tol
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("order", [21, 42])
+@pytest.mark.parametrize("order", [21, 42, 63])
 def test_ts2_config_matches_oracle(oracle_mod, tmp_path, order):
     """run_ts2_linear.sh: rotated axis [1,0,1]/sqrt(2), Earth's omega, linear
     Williamson 2 (a steady state of the linear operator), 2 hours at dt=900."""

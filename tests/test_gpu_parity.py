@@ -382,11 +382,30 @@ def test_integrals_diag_matches_oracle(oracle_mod, order, N, precision, omega):
     enstrophy) against the oracle on a random state over a deep layer: FP64
     1e-12 relative, FP32 2e-6; omega = 0 leaves the enstrophy to the curl
     alone (its SAT halos included)."""
-    _say(f"integrals_diag order={order} N={N} precision={precision} omega={omega}")
+    _check_diag(oracle_mod, order, N, precision, omega, False)
+
+
+# the multi-kernel diagnostics (unfused plans, and every sbp63 plan): N = 12,
+# 13 put the sbp63 closures of both panel ends next to each other
+MK_DIAG_CASES = [(21, 20, cs.FP64, 0.0), (42, 33, cs.FP64, 7.292e-5), (63, 12, cs.FP64, 0.0),
+                 (63, 13, cs.FP64, 7.292e-5), (63, 40, cs.FP64, 7.292e-5), (63, 32, cs.FP32, 0.0)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,N,precision,omega", MK_DIAG_CASES)
+def test_multi_kernel_diagnostics_match_oracle(oracle_mod, order, N, precision, omega):
+    """The same sums on the multi-kernel path: integrals, CFL and the
+    integral tendencies."""
+    _check_diag(oracle_mod, order, N, precision, omega, True)
+    _check_tends(oracle_mod, order, N, precision, omega, True)
+
+
+def _check_diag(oracle_mod, order, N, precision, omega, unfused):
+    _say(f"integrals_diag order={order} N={N} precision={precision} omega={omega} unfused={unfused}")
     o = oracle_mod.Oracle(N, order, radius=6371229.0, omega=omega)
     u, v, h = _random_state(o, 40 + N)
     h = 500.0 + 10.0 * h
-    op = _op(N, order, precision, radius=6371229.0, omega=omega)
+    op = _op(N, order, precision, radius=6371229.0, omega=omega, unfused=unfused)
     st = op.create_stvec().upload(u, v, h)
     got = op.integrals_diag(st, 9.80616)
     ref = o.integrals_diag(u, v, h, 9.80616)
@@ -490,11 +509,15 @@ def test_integral_tends_match_oracle(oracle_mod, order, N, precision, omega):
     winds of 10 m/s over a deep layer.  Both results are sums that cancel, so
     the error is measured against the sum of their terms' absolute values:
     FP64 1e-12, FP32 2e-6 of it."""
-    _say(f"integral_tends order={order} N={N} precision={precision} omega={omega}")
+    _check_tends(oracle_mod, order, N, precision, omega, False)
+
+
+def _check_tends(oracle_mod, order, N, precision, omega, unfused):
+    _say(f"integral_tends order={order} N={N} precision={precision} omega={omega} unfused={unfused}")
     o = oracle_mod.Oracle(N, order, radius=6371229.0, omega=omega)
     u, v, h = _random_state(o, 60 + N)
     u, v, h = 10.0 * u, 10.0 * v, 500.0 + 10.0 * h
-    op = _op(N, order, precision, radius=6371229.0, omega=omega)
+    op = _op(N, order, precision, radius=6371229.0, omega=omega, unfused=unfused)
     st = op.create_stvec().upload(u, v, h)
     got = op.integral_tends(st, 9.80616)
     te, en, ste, sen = o.integral_tends(u, v, h, 9.80616, scales=True)

@@ -119,15 +119,3 @@ def test_energy_is_conserved_without_rotation(oracle_mod):
     e0 = op.integrals(st)[1]
     cs.create_timescheme("rk4").step(st, op, 300.0, 40)
     assert abs(op.integrals(st)[1] / e0 - 1) < 1e-7
-
-
-def test_fused_only_entries_refuse_sbp63():
-    """The diagnostics kernels are the fused path's: an sbp63 plan refuses
-    them with a message rather than computing them with other closures."""
-    _say("sbp63 refusals")
-    op = cs.create_linear_swm_operator(16, 63, cs.FP64)
-    st = op.create_stvec()
-    for call in (lambda: op.integrals_diag(st), lambda: op.cfl_diag(st, 100.0),
-                 lambda: op.integral_tends(st)):
-        with pytest.raises(cs.SweError, match="fused plans only"):
-            call()
