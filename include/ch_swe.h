@@ -144,7 +144,9 @@ int swe_integrals_noalloc(swe_plan *plan, swe_state *st, double *mass, double *e
  * co2contra_ch_sbpXX, SBP_CXX quadratures), covariant winds, no orography.
  * out[7] = hmin, hmax, mass, TE, KE, PE, enstrophy (FP64 sums on the device);
  * grav is the PE's gravity (the reference uses Earth_grav there whatever the
- * operator's).  Fused plans only; blocks until the values are on the host. */
+ * operator's).  Fused sbp21 / sbp42 plans sum tile by tile in one launch,
+ * others (SWE_FLAG_UNFUSED, sbp63) point by point on the multi-kernel path;
+ * blocks until the values are on the host. */
 int swe_integrals_diag(swe_plan *plan, swe_state *st, double grav, double out[7]);
 
 /* operator_swm_lin_t%solve
@@ -215,8 +217,8 @@ int swe_latlon_uv(swe_plan *plan, swe_state *st, int nlat, int nlon, float *u_ou
  * the plan's operator: the tendency apply(st) goes to a plan-owned buffer
  * (allocated on the first call), then out[2] = TE_tend, Enstrophy_tend with
  * the same operators as swe_integrals_diag; hq is the first output of
- * interp2d_uv2qvec (i2c along y of h at the u points), as there.  Fused
- * plans only; blocks until the values are on the host. */
+ * interp2d_uv2qvec (i2c along y of h at the u points), as there.  Blocks
+ * until the values are on the host. */
 int swe_integral_tends(swe_plan *plan, swe_state *st, double grav, double out[2]);
 
 /* Serving: continuous batching of independent solves of mixed size and
