@@ -349,6 +349,41 @@ def bench_serving(args, ev):
     }
 
 
+def bench_implicit(args):
+    """The implicit solve (swe_solve: CG on the Helmholtz problem for h,
+    operator_swm_lin_mod.f90 L110-146) at CFL 4 on a seeded random
+    right-hand side, rel_tol 1e-8: microseconds per CG iteration for N = 96
+    and N = args.N, FP64 and FP32 (wall time of one solve after a warm-up
+    solve, the solve ends synchronised)."""
+    import ch_swe_amd as cs
+    rows = []
+    for N, prec in ((96, cs.FP64), (args.N, cs.FP64), (args.N, cs.FP32)):
+        op = cs.create_linear_swm_operator(N, ORDER, prec, radius=6371229.0)
+        rng = np.random.default_rng(0)
+        r = op.create_stvec().upload(rng.standard_normal((6, N + 1, N)), rng.standard_normal((6, N, N + 1)),
+                                     rng.standard_normal((6, N + 1, N + 1)))
+        x = op.create_stvec()
+        dt = 4 * (0.5 * np.pi / N) * 6371229.0 / np.sqrt(9.81 * 1000.0)
+        op.solve(x, r, dt, rel_tol=1e-8, max_iter=300)
+        op.synchronize()
+        t0 = time.perf_counter()
+        it, rel = op.solve(x, r, dt, rel_tol=1e-8, max_iter=300)
+        op.synchronize()
+        t = time.perf_counter() - t0
+        rows.append({"N": N, "dtype": "f64" if prec == cs.FP64 else "f32", "cg_iterations": int(it),
+                     "rel_residual": float(rel), "us_per_iteration": round(t * 1e6 / max(it, 1), 1)})
+        print(f"# implicit solve {rows[-1]}", file=sys.stderr, flush=True)
+    top = rows[-1]
+    return {
+        "metric": "CG iterations per second of the implicit solve (Helmholtz problem, CFL 4, rel_tol 1e-8)",
+        "value": round(1e6 / top["us_per_iteration"], 1), "unit": "iterations/s", "n_gpus": 1, "steps": None,
+        "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic seeded random right-hand side",
+        "config": {"workload": f"implicit solve: N={args.N} sbp42 FP32 (rows: N=96 FP64, N={args.N} FP64 / FP32)"},
+        "per_case": rows,
+    }
+
+
 def accuracy_leg(cs, tc, N, state0, dt, steps):
     """FP32 (plain and with compensated summation) against FP64 after
     `steps` RK4 steps from the benchmark state, untimed: relative SBP-L2
@@ -432,8 +467,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=host_threads())
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", choices=["single", "ensemble", "uneven", "serving"], default="single",
-                    help="single large solve (default, the headline), uniform ensemble, uneven batch, serving")
+    ap.add_argument("--workload", choices=["single", "ensemble", "uneven", "serving", "implicit"], default="single",
+                    help="single large solve (default, the headline), uniform ensemble, uneven batch, serving, "
+                         "implicit solve")
     ap.add_argument("--rates", default="5,10,20,40", help="serving: request rates (per second)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serving: device-memory budget")
     ap.add_argument("--chunk", type=int, default=25, help="serving: steps per tick")
@@ -459,6 +495,8 @@ def main():
             line = bench_ensemble(args, ev)
         elif args.workload == "serving":
             line = bench_serving(args, ev)
+        elif args.workload == "implicit":
+            line = bench_implicit(args)
         else:
             args.members = args.members or 64
             line = bench_uneven(args, ev)

@@ -264,8 +264,20 @@ __global__ void k_axpy(long n, T *__restrict__ out, T s1, const T *__restrict__ 
 
 // sum over the interior xy points of w a b (w: the SBP_C quadrature weight),
 // FP64 partials per block in a fixed order
-template <typename T, int O>
-__global__ __launch_bounds__(256) void k_xy_dot(Dims d, double hx, const T *a, const T *b, double *part)
+// the xy quadrature weights of k_xy_dot, once per plan (FP64, state layout)
+template <int O>
+__global__ void k_xy_weights(Dims d, double hx, double *w)
+{
+    const int n = d.N + 1;
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= 6L * n * n) return;
+    int p, i, j;
+    unflat(k, n, n, p, i, j);
+    w[off(d, p, i, j)] = quad_weight(d.N, O, hx, 0, i, j);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_xy_dot(Dims d, const double *w, const T *a, const T *b, double *part)
 {
     __shared__ double ws[4];
     const int n = d.N + 1;
@@ -275,7 +287,7 @@ __global__ __launch_bounds__(256) void k_xy_dot(Dims d, double hx, const T *a, c
         int p, i, j;
         unflat(k, n, n, p, i, j);
         const long o = off(d, p, i, j);
-        s += quad_weight(d.N, O, hx, 0, i, j) * (double)a[o] * (double)b[o];
+        s += w[o] * (double)a[o] * (double)b[o];
     }
     for (int m = 32; m >= 1; m >>= 1) s += __shfl_down(s, m);
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
@@ -283,12 +295,16 @@ __global__ __launch_bounds__(256) void k_xy_dot(Dims d, double hx, const T *a, c
     if (threadIdx.x == 0) part[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
+// one wave: lane l sums partials l, l + 64, ... in order, then a fixed
+// shuffle tree (deterministic; one thread walking all partials costs the
+// serial latency of nb loads)
 __global__ void k_dot_final(const double *part, int nb, double *out)
 {
-    if (threadIdx.x != 0) return;
+    const int l = threadIdx.x & 63;
     double s = 0;
-    for (int b = 0; b < nb; b++) s += part[b];
-    *out = s;
+    for (int b = l; b < nb; b += 64) s += part[b];
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_down(s, m);
+    if (threadIdx.x == 0) *out = s;
 }
 
 __global__ void k_fill(double *p, int n, double v)
@@ -316,9 +332,13 @@ __global__ void k_cg_step(long n, T *x, T *r, const T *p, const T *Ap, const dou
 }
 
 // the relative residual of iteration it against the tolerance (one thread)
-__global__ void k_cg_check(const double *rr_new, const double *bb, double rel_tol, int it, double *st)
+__global__ void k_cg_check(const double *rr_new, const double *bb, double rel_tol, double *st)
 {
     if (threadIdx.x != 0 || st[0] != 0.0) return;
+    // st[3] counts the iterations, so that every batch of iterations is the
+    // same launch sequence (replayed from one graph)
+    const double it = st[3] + 1.0;
+    st[3] = it;
     const double rel = sqrt(*rr_new) / sqrt(*bb);
     st[2] = rel;
     if (rel < rel_tol) {
@@ -724,6 +744,7 @@ struct Engine : EngineBase {
     ~Engine() override
     {
         if (graph) (void)hipGraphExecDestroy(graph);
+        if (cg_graph) (void)hipGraphExecDestroy(cg_graph);
         for (void *p : allocs) (void)hipFree(p);
     }
 
@@ -1230,14 +1251,12 @@ struct Engine : EngineBase {
     // the first call)
     T *sv_s = nullptr, *sv_t1 = nullptr, *sv_t2 = nullptr, *sv_r = nullptr, *sv_p = nullptr, *sv_ap = nullptr,
       *sv_b = nullptr;
-    double *sv_part = nullptr, *sv_sc = nullptr;
+    double *sv_part = nullptr, *sv_sc = nullptr, *sv_w = nullptr;  // sv_w: xy quadrature weights
     static constexpr int DOT_BLOCKS = 256;
     static constexpr int CG_BATCH = 8;  // CG iterations between convergence read-backs
     void xy_dot(const T *a, const T *b, double *out)
     {
-        if (order == 21) k_xy_dot<T, 21><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, geo.hx, a, b, sv_part);
-        else if (order == 42) k_xy_dot<T, 42><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, geo.hx, a, b, sv_part);
-        else k_xy_dot<T, 63><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, geo.hx, a, b, sv_part);
+        k_xy_dot<T><<<DOT_BLOCKS, 256, 0, pl->stream>>>(d, sv_w, a, b, sv_part);
         k_dot_final<<<1, 64, 0, pl->stream>>>(sv_part, DOT_BLOCKS, out);
     }
     // swm_helm_lin_operator_t%apply: az = z - gamma div co2contra grad z with
@@ -1252,6 +1271,59 @@ struct Engine : EngineBase {
         k_axpy<T><<<blocks_for(d.F), TPB, 0, pl->stream>>>(d.F, az, T(1.0), z, T(-(dt * dt)), sv_t2 + 2 * d.F, 0);
         return SWE_OK;
     }
+    // one CG iteration (cg_solver_mod.f90 L48-88); parity: which of the slots
+    // 0 / 2 holds |r|^2 before the step (they alternate); every kernel
+    // after convergence is a no-op (the state's flag)
+    int cg_iter(T *x, double dt, double rel_tol, double *cst, int parity)
+    {
+        const long F = d.F;
+        const int a = parity ? 2 : 0, b = parity ? 0 : 2;
+        int rc;
+        if ((rc = helm(sv_p, sv_ap, dt))) return rc;
+        xy_dot(sv_p, sv_ap, sv_sc + 1);
+        k_cg_step<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, x, sv_r, sv_p, sv_ap, sv_sc + a, sv_sc + 1, cst);
+        xy_dot(sv_r, sv_r, sv_sc + b);
+        k_cg_check<<<1, 64, 0, pl->stream>>>(sv_sc + b, sv_sc + 3, rel_tol, cst);
+        k_cg_dir<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, sv_p, sv_r, sv_sc + b, sv_sc + a, cst);
+        return SWE_OK;
+    }
+    // CG_BATCH iterations (an even number: the slots are back in place
+    // after it) from a graph captured for the last (x, dt, rel_tol); the
+    // iterations launch about a dozen small kernels each, whose launch gaps
+    // bound the solve at small N
+    hipGraphExec_t cg_graph = nullptr;
+    const void *cg_key = nullptr;
+    double cg_dt = 0, cg_tol = 0;
+    static_assert(CG_BATCH % 2 == 0, "a CG batch must leave the |r|^2 slots in place");
+    int cg_batch(T *x, double dt, double rel_tol, double *cst)
+    {
+        if (!cg_graph || cg_key != x || cg_dt != dt || cg_tol != rel_tol) {
+            if (cg_graph) (void)hipGraphExecDestroy(cg_graph);
+            cg_graph = nullptr;
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal));
+            int rc = SWE_OK;
+            for (int q = 0; q < CG_BATCH && rc == SWE_OK; q++) rc = cg_iter(x, dt, rel_tol, cst, q & 1);
+            const hipError_t le = hipGetLastError();
+            const hipError_t ce = hipStreamEndCapture(pl->stream, &g);
+            if (rc || le != hipSuccess || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                if (rc) return rc;
+                return fail(SWE_EHIP, std::string("CG graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+            }
+            const hipError_t ie = hipGraphInstantiate(&cg_graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                cg_graph = nullptr;
+                return fail(SWE_EHIP, std::string("CG graph instantiate: ") + hipGetErrorString(ie));
+            }
+            cg_key = x;
+            cg_dt = dt;
+            cg_tol = rel_tol;
+        }
+        HIP_TRY(hipGraphLaunch(cg_graph, pl->stream));
+        return SWE_OK;
+    }
     int solve(void *vout, void *rhs, double dt, int max_iter, double rel_tol, int *iters, double *relres) override
     {
         int rc;
@@ -1263,8 +1335,13 @@ struct Engine : EngineBase {
             for (T **q : f1)
                 if ((rc = field(q))) return rc;
             if ((rc = alloc((void **)&sv_part, DOT_BLOCKS * sizeof(double))) ||
-                (rc = alloc((void **)&sv_sc, 8 * sizeof(double))))
+                (rc = alloc((void **)&sv_sc, 8 * sizeof(double))) ||
+                (rc = alloc((void **)&sv_w, d.F * sizeof(double))))
                 return rc;
+            const long nxy = 6L * (d.N + 1) * (d.N + 1);
+            if (order == 21) k_xy_weights<21><<<blocks_for(nxy), TPB, 0, pl->stream>>>(d, geo.hx, sv_w);
+            else if (order == 42) k_xy_weights<42><<<blocks_for(nxy), TPB, 0, pl->stream>>>(d, geo.hx, sv_w);
+            else k_xy_weights<63><<<blocks_for(nxy), TPB, 0, pl->stream>>>(d, geo.hx, sv_w);
         }
         T *out = (T *)vout, *in = (T *)rhs, *x = out + 2 * d.F;
         const long F = d.F;
@@ -1287,29 +1364,27 @@ struct Engine : EngineBase {
         if (!(rel < rel_tol)) {
             HIP_TRY(hipMemcpyAsync(sv_p, sv_r, F * sizeof(T), hipMemcpyDeviceToDevice, pl->stream));
             double *cst = sv_sc + 4;  // convergence state (k_cg_check)
-            k_fill<<<1, 64, 0, pl->stream>>>(cst, 3, 0.0);
-            int a = 0, b = 2;  // slots of |r|^2 before and after the step
+            k_fill<<<1, 64, 0, pl->stream>>>(cst, 4, 0.0);
             bool done = false;
             it = max_iter + 1;  // not converged (also when max_iter is 0)
-            for (int k = 1; k <= max_iter && !done; k++) {
-                if ((rc = helm(sv_p, sv_ap, dt))) return rc;
-                xy_dot(sv_p, sv_ap, sv_sc + 1);
-                k_cg_step<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, x, sv_r, sv_p, sv_ap, sv_sc + a, sv_sc + 1,
-                                                                     cst);
-                xy_dot(sv_r, sv_r, sv_sc + b);
-                k_cg_check<<<1, 64, 0, pl->stream>>>(sv_sc + b, sv_sc + 3, rel_tol, k, cst);
-                k_cg_dir<T><<<blocks_for(F), TPB, 0, pl->stream>>>(F, sv_p, sv_r, sv_sc + b, sv_sc + a, cst);
-                std::swap(a, b);
-                // the host looks at the convergence state every CG_BATCH
-                // iterations and after the last
-                if (k % CG_BATCH == 0 || k == max_iter) {
-                    double cs[3];
-                    HIP_TRY(hipMemcpyAsync(cs, cst, 3 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
-                    HIP_TRY(hipStreamSynchronize(pl->stream));
-                    done = cs[0] != 0.0;
-                    rel = cs[2];
-                    it = done ? (int)cs[1] : k + 1;
+            // batches of CG_BATCH iterations (replayed from a graph), the
+            // rest one by one; the host looks at the convergence state after
+            // each batch and after the last iteration
+            for (int k = 0; k < max_iter && !done;) {
+                const int nb = std::min(CG_BATCH, max_iter - k);
+                if (nb == CG_BATCH && !timer.on) {
+                    if ((rc = cg_batch(x, dt, rel_tol, cst))) return rc;
+                } else {
+                    for (int q = 0; q < nb; q++)
+                        if ((rc = cg_iter(x, dt, rel_tol, cst, (k + q) & 1))) return rc;
                 }
+                k += nb;
+                double cs[3];
+                HIP_TRY(hipMemcpyAsync(cs, cst, 3 * sizeof(double), hipMemcpyDeviceToHost, pl->stream));
+                HIP_TRY(hipStreamSynchronize(pl->stream));
+                done = cs[0] != 0.0;
+                rel = cs[2];
+                it = done ? (int)cs[1] : k + 1;
             }
         }
         if (iters) *iters = it;
