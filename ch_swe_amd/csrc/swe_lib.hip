@@ -397,13 +397,18 @@ __global__ void k_reduce(Dims d, const T *h, const T *u, const T *v, const T *ut
         for (int q = 0; q < 3; q++) part[3 * blockIdx.x + q] = sm[q][0];
 }
 
+// one wave, fixed order: lane l sums blocks l, l + 64, ..., then a shuffle
+// tree (one thread walking nb blocks costs nb serial loads)
 __global__ void k_reduce_final(const double *part, int nb, double *out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int l = threadIdx.x & 63;
     double s[3] = {0, 0, 0};
-    for (int k = 0; k < nb; k++)
+    for (int k = l; k < nb; k += 64)
         for (int q = 0; q < 3; q++) s[q] += part[3 * k + q];
-    for (int q = 0; q < 3; q++) out[q] = s[q];
+    for (int q = 0; q < 3; q++)
+        for (int m = 32; m >= 1; m >>= 1) s[q] += __shfl_down(s[q], m);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; q++) out[q] = s[q];
 }
 
 #include "swe_fused.inc"

@@ -491,7 +491,7 @@ struct Plan {
         const double dx = RADIUS * g.hx;
         emu::launch(grid, F_THREADS,
                     [&] { k_diag<T, O, TY>(d, cf, g, st, ft.ntx, ft.nty, ft.fwx, ft.fwy, grav, 1.0 / (dx * dx), pp); });
-        emu::launch_flat(1, 64, [&] { k_diag_final(pp, grid, op); });
+        emu::launch(1, 64, [&] { k_diag_final(pp, grid, op); });
         for (int k = 0; k < 7; k++) out[k] = o[k];
     }
 
@@ -561,7 +561,7 @@ struct Plan {
         double *pp = prt.data(), *op = o.data();
         emu::launch(grid, F_THREADS,
                     [&] { k_tends<T, O, TY>(d, cf, g, st, tend, ft.ntx, ft.nty, ft.fwx, ft.fwy, grav, pp); });
-        emu::launch_flat(1, 64, [&] { k_tends_final(pp, grid, op); });
+        emu::launch(1, 64, [&] { k_tends_final(pp, grid, op); });
         out[0] = o[0];
         out[1] = o[1];
     }
