@@ -703,6 +703,11 @@ struct swe_plan {
     std::unique_ptr<EngineBase> eng;
     swe_state *rk[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // k1..k4, y
     swe_state *imp = nullptr;  // v_new of the implicit Euler and Crank-Nicolson steps
+    // the multi-kernel RK4 loop's graph of MK_GRAPH_STEPS steps, for the last
+    // (state, dt) it was captured for
+    hipGraphExec_t mk_graph = nullptr;
+    const void *mk_key = nullptr;
+    double mk_dt = 0;
 };
 
 struct swe_state {
@@ -1664,6 +1669,7 @@ int swe_plan_destroy(swe_plan *plan)
     }
     swe_state_destroy(plan->imp);
     plan->imp = nullptr;
+    if (plan->mk_graph) (void)hipGraphExecDestroy(plan->mk_graph);
     plan->eng.reset();
     if (plan->stream) (void)hipStreamDestroy(plan->stream);
     delete plan;
@@ -1801,18 +1807,49 @@ int swe_rk4_steps(swe_plan *plan, swe_state *v0, double dt, int nsteps)
         if (!plan->rk[k] && (rc = swe_state_create(plan, &plan->rk[k]))) return rc;
     swe_state *k1 = plan->rk[0], *k2 = plan->rk[1], *k3 = plan->rk[2], *k4 = plan->rk[3], *y = plan->rk[4];
     if (plan->eng->fused()) return plan->eng->rk4(v0->buf, k1->buf, k2->buf, k3->buf, dt, nsteps);
-    for (int it = 0; it < nsteps; it++) {
-        // rk4_mod.f90 step_rk4 L28-40
-        if ((rc = swe_apply(plan, k1, v0))) return rc;
-        if ((rc = swe_assign(plan, y, 1.0, v0, 0.5 * dt, k1))) return rc;
-        if ((rc = swe_apply(plan, k2, y))) return rc;
-        if ((rc = swe_assign(plan, y, 1.0, v0, 0.5 * dt, k2))) return rc;
-        if ((rc = swe_apply(plan, k3, y))) return rc;
-        if ((rc = swe_assign(plan, y, 1.0, v0, dt, k3))) return rc;
-        if ((rc = swe_apply(plan, k4, y))) return rc;
-        if ((rc = swe_update(plan, v0, dt / 6.0, k1, dt / 3.0, k2))) return rc;
-        if ((rc = swe_update(plan, v0, dt / 3.0, k3, dt / 6.0, k4))) return rc;
+    // rk4_mod.f90 step_rk4 L28-40: about 45 launches a step on the
+    // multi-kernel path, so chunks of MK_GRAPH_STEPS steps are replayed from a
+    // captured graph (no memset nodes); the rest and timed runs launch directly
+    auto step = [&]() -> int {
+        int r;
+        if ((r = swe_apply(plan, k1, v0))) return r;
+        if ((r = swe_assign(plan, y, 1.0, v0, 0.5 * dt, k1))) return r;
+        if ((r = swe_apply(plan, k2, y))) return r;
+        if ((r = swe_assign(plan, y, 1.0, v0, 0.5 * dt, k2))) return r;
+        if ((r = swe_apply(plan, k3, y))) return r;
+        if ((r = swe_assign(plan, y, 1.0, v0, dt, k3))) return r;
+        if ((r = swe_apply(plan, k4, y))) return r;
+        if ((r = swe_update(plan, v0, dt / 6.0, k1, dt / 3.0, k2))) return r;
+        return swe_update(plan, v0, dt / 3.0, k3, dt / 6.0, k4);
+    };
+    constexpr int MK_GRAPH_STEPS = 16;
+    int done = 0;
+    if (!plan->eng->timer.on && nsteps >= MK_GRAPH_STEPS) {
+        if (!plan->mk_graph || plan->mk_key != v0->buf || plan->mk_dt != dt) {
+            if (plan->mk_graph) (void)hipGraphExecDestroy(plan->mk_graph);
+            plan->mk_graph = nullptr;
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(plan->stream, hipStreamCaptureModeThreadLocal));
+            for (int it = 0; it < MK_GRAPH_STEPS && rc == SWE_OK; it++) rc = step();
+            const hipError_t ce = hipStreamEndCapture(plan->stream, &g);
+            if (rc || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc ? rc : fail(SWE_EHIP, std::string("RK4 graph capture: ") + hipGetErrorString(ce));
+            }
+            const hipError_t ie = hipGraphInstantiate(&plan->mk_graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                plan->mk_graph = nullptr;
+                return fail(SWE_EHIP, std::string("RK4 graph instantiate: ") + hipGetErrorString(ie));
+            }
+            plan->mk_key = v0->buf;
+            plan->mk_dt = dt;
+        }
+        for (; done + MK_GRAPH_STEPS <= nsteps; done += MK_GRAPH_STEPS)
+            HIP_TRY(hipGraphLaunch(plan->mk_graph, plan->stream));
     }
+    for (; done < nsteps; done++)
+        if ((rc = step())) return rc;
     return SWE_OK;
 }
 

@@ -119,3 +119,24 @@ def test_energy_is_conserved_without_rotation(oracle_mod):
     e0 = op.integrals(st)[1]
     cs.create_timescheme("rk4").step(st, op, 300.0, 40)
     assert abs(op.integrals(st)[1] / e0 - 1) < 1e-7
+
+
+@pytest.mark.parametrize("order", [63, 42])
+def test_multi_kernel_rk4_graph_replays_match_oracle(oracle_mod, order):
+    """The multi-kernel RK4 loop replays chunks of 16 steps from a captured
+    graph: 40 steps (two replays and 8 direct steps), then 33 steps with
+    another dt (a new graph) on the same state, against the oracle."""
+    N = 14
+    _say(f"multi-kernel rk4 graph replays order={order}")
+    o = oracle_mod.Oracle(N, order)
+    u, v, h = _random_state(o, 5)
+    u, v = 10 * u, 10 * v
+    ref = o.rk4(u, v, h, 300.0, 40)
+    ref = o.rk4(*ref, 200.0, 33)
+    op = cs.create_linear_swm_operator(N, order, cs.FP64, unfused=True)
+    st = op.create_stvec().upload(u, v, h)
+    ts = cs.create_timescheme("rk4")
+    ts.step(st, op, 300.0, 40)
+    ts.step(st, op, 200.0, 33)
+    for g, r in zip(st.download(), ref):
+        assert _rel(g, r) < 1e-12
